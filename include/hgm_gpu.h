@@ -1,0 +1,255 @@
+/*
+ * hgm_gpu.h -- C ABI of the MI355X (gfx950) t_GPU execution library.
+ *
+ * This is the drop-in boundary for HybriD-GM's GPU execution path. It has two
+ * layers:
+ *
+ *   1. The reference-compatible entry points, with the exact names, argument
+ *      order and meaning that the reference's DGM binds
+ *      (include/core/dgm.h:22-32, defined in src/core/kernel.cu:31,303,321,384).
+ *      A reference build links this library in place of kernel.o.
+ *
+ *   2. A POD ABI (hgm_*) with plain pointers, sizes and status codes, for
+ *      callers that do not carry the reference's C++ PT struct: op arrays,
+ *      device-resident states, plans and a DGM handle.
+ *
+ * Amplitude layout everywhere: interleaved (re, im) pairs, index bit
+ * (qubits-1-q) <-> circuit qubit q (qubit 0 is the most significant bit,
+ * src/core/dgm_parser.cpp:151-152). Gate matrices are row-major
+ * [m00, m01, m10, m11] and every gate maps a pair (a0, a1) differing in the
+ * target bit to (m00*a0 + m01*a1, m10*a0 + m11*a1) (src/core/kernel.cu:109-111).
+ *
+ * Errors: every hgm_* call returns HGM_OK (0) or an HGM_E_* code and records
+ * a message readable through hgm_last_error() (per calling thread). The
+ * reference-compatible entry points keep the reference's convention instead:
+ * print the message and exit(1) (src/core/kernel.cu:439-447).
+ */
+#ifndef HGM_GPU_H
+#define HGM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+typedef __complex__ float hgm_cfloat;
+#else
+#include <stdbool.h>
+typedef float _Complex hgm_cfloat;
+#endif
+
+/* ---------------------------------------------------------------- codes */
+enum {
+    HGM_OK = 0,
+    HGM_E_ARG = 1,         /* bad argument (null pointer, size, qubit index) */
+    HGM_E_TUNING = 2,      /* tuning tuple violates the contract */
+    HGM_E_HIP = 3,         /* HIP runtime error (message carries hipGetErrorString) */
+    HGM_E_NOMEM = 4,       /* device or pinned host allocation failed */
+    HGM_E_PARSE = 5,       /* circuit text names an unknown gate or is malformed */
+    HGM_E_STATE = 6,       /* state handle / qubit count mismatch */
+    HGM_E_UNSUPPORTED = 7  /* feature outside this library (t_CPU, t_HYBRID, multi-GPU) */
+};
+
+/* amplitude types */
+enum { HGM_C128 = 0, /* complex double, 16 B/amplitude (default) */
+       HGM_C64 = 1   /* complex float, 8 B/amplitude (the reference's precision) */ };
+
+/* pass planners */
+enum { HGM_PLAN_GREEDY = 0, /* dependency-aware lookahead, any set of tile-local bits */
+       HGM_PLAN_WINDOW = 1  /* in-order, contiguous window (the contract of src/core/kernel.cu:199-235) */ };
+
+/* kernel families */
+enum { HGM_KERNEL_AUTO = 0, HGM_KERNEL_LDS = 1, HGM_KERNEL_REG = 2 };
+
+/* One compiled single-qubit gate: the POD form of the reference's PT
+ * (include/core/common.h:57-90). */
+typedef struct hgm_op {
+    int32_t target_bit;      /* state-index bit the gate acts on */
+    int32_t reserved;        /* must be 0 */
+    uint64_t control_mask;   /* state-index bits that must match control_value */
+    uint64_t control_value;
+    double m[8];             /* m00.re m00.im m01.re m01.im m10.re m10.im m11.re m11.im */
+} hgm_op;
+
+/* The tuning contract of the reference (include/cli/cli_common.h:8-17,
+ * src/core/dgm_core.cpp:177-241, makefile:7): a pass loads tiles of
+ * 2^gpu_region_bits amplitudes whose low gpu_coalesced_bits index bits are
+ * contiguous in memory; block_size threads each own 2*repeat_count of them;
+ * at most ops_block gates are fused into one pass.
+ * Invariant: 2*block_size*repeat_count == 2^gpu_region_bits. */
+typedef struct hgm_tuning {
+    int32_t block_size;          /* threads per workgroup (reference default 64) */
+    int32_t repeat_count;        /* amplitude pairs per thread (default 2) */
+    int32_t gpu_region_bits;     /* log2 tile size (default 8) */
+    int32_t gpu_coalesced_bits;  /* low index bits always in the tile (default 4) */
+    int32_t ops_block;           /* max gates per pass (OPS_BLOCK, default 300) */
+    int32_t planner;             /* HGM_PLAN_* */
+    int32_t kernel;              /* HGM_KERNEL_* */
+    int32_t lookahead;           /* greedy planner scan window in gates (0 = default 4096) */
+} hgm_tuning;
+
+/* Per-call statistics. bytes_alg = sum over passes of 2 * tile amplitudes *
+ * sizeof(amplitude) = the HBM bytes one read and one write of every loaded
+ * amplitude costs. */
+typedef struct hgm_stats {
+    int64_t gates;           /* gates applied (iterations included) */
+    int64_t passes;          /* kernel launches over the state */
+    int64_t amp_updates;     /* gates * 2^qubits */
+    double bytes_alg;        /* algorithmic HBM bytes of all passes */
+    double plan_ms;          /* host planning time */
+    double h2d_ms;           /* host-to-device copy (0 for resident states) */
+    double kernel_ms;        /* HIP-event time over all passes, on the launch stream */
+    double d2h_ms;           /* device-to-host copy */
+    double total_ms;         /* wall time of the call */
+} hgm_stats;
+
+/* Default tuning (the reference's CLI defaults, greedy planner, auto kernel). */
+void hgm_default_tuning(hgm_tuning *t);
+
+/* Last error message of the calling thread ("" if none). */
+const char *hgm_last_error(void);
+
+/* Library version string. */
+const char *hgm_version(void);
+
+/* ------------------------------------------------------------------ device */
+/* Select and initialise device `device` (creates the context and the stream
+ * the library launches on). Idempotent. */
+int hgm_init(int device);
+/* Number of visible HIP devices (0 when none). */
+int hgm_device_count(int *count);
+/* Free cached device buffers held for hgm_execute. */
+int hgm_release_cache(void);
+
+/* ------------------------------------------------------------------ one-shot
+ * Applies ops[0..n_ops) `iterations` times to a host state of 2^qubits
+ * amplitudes of type dtype, in place: upload, run every pass, download. This
+ * is what the reference's t_GPU does per execute() (src/core/kernel.cu:131-291),
+ * with the device buffer cached across calls. */
+int hgm_execute(void *host_state, int qubits, int dtype, const hgm_op *ops, int64_t n_ops,
+                const hgm_tuning *tuning, int iterations, hgm_stats *stats);
+
+/* ------------------------------------------------------------------ resident states */
+typedef struct hgm_state hgm_state;
+
+/* Allocate a device-resident state of 2^qubits amplitudes (zeroed). */
+int hgm_state_create(int qubits, int dtype, hgm_state **out);
+/* Borrow caller-owned device memory (e.g. a framework tensor); not freed by destroy. */
+int hgm_state_wrap(void *device_ptr, int qubits, int dtype, hgm_state **out);
+void hgm_state_destroy(hgm_state *s);
+int hgm_state_qubits(const hgm_state *s);
+int hgm_state_dtype(const hgm_state *s);
+void *hgm_state_device_ptr(hgm_state *s);
+/* |index> on the device, no host copy (setMemoryValue, src/core/dgm_core.cpp:63-65). */
+int hgm_state_set_basis(hgm_state *s, uint64_t index);
+int hgm_state_upload(hgm_state *s, const void *host);
+int hgm_state_download(const hgm_state *s, void *host);
+/* Apply a gate list `iterations` times (plans, uploads the op tables once, launches). */
+int hgm_state_apply(hgm_state *s, const hgm_op *ops, int64_t n_ops, const hgm_tuning *tuning,
+                    int iterations, hgm_stats *stats);
+/* Sum of |a|^2 over the whole state (FP64 accumulation on the device). */
+int hgm_state_norm2(const hgm_state *s, double *out);
+/* P(qubit == 1) (FP64 accumulation on the device). qubit counts from the MSB. */
+int hgm_state_prob_one(const hgm_state *s, int qubit, double *out);
+/* Collapse qubit to `outcome` and move the surviving half to the |0> slot,
+ * dividing by sqrt(p) (src/core/dgm_core.cpp:115-122). */
+int hgm_state_collapse_reset(hgm_state *s, int qubit, int outcome, double p);
+/* Index of the largest |a|^2 (ties -> lowest index) and its probability. */
+int hgm_state_argmax(const hgm_state *s, uint64_t *index, double *prob);
+
+/* ------------------------------------------------------------------ plans (CPU only) */
+typedef struct hgm_plan hgm_plan;
+int hgm_plan_build(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations,
+                   const hgm_tuning *tuning, hgm_plan **out);
+void hgm_plan_destroy(hgm_plan *p);
+int64_t hgm_plan_num_passes(const hgm_plan *p);
+/* For pass i: the tile-local bit mask, the number of gates, and (optionally)
+ * their indices into the (iteration-expanded) op list, in execution order. */
+int hgm_plan_pass(const hgm_plan *p, int64_t i, uint64_t *local_mask, int64_t *n_gates, int64_t *gate_index);
+/* Effective tile shape of a plan: region bits, block size, repeat count, coalesced bits. */
+int hgm_plan_shape(const hgm_plan *p, int32_t *region_bits, int32_t *block_size, int32_t *repeat_count,
+                   int32_t *coalesced_bits);
+int hgm_state_run_plan(hgm_state *s, const hgm_plan *p, hgm_stats *stats);
+
+/* Tuning check with the reference's rules plus the gfx950 limits; returns
+ * HGM_E_TUNING with a message when the tuple is rejected. */
+int hgm_validate_tuning(int qubits, int dtype, const hgm_tuning *t);
+
+/* ------------------------------------------------------------------ DGM handle
+ * The executor API of the reference (class DGM, include/core/dgm.h:65-163)
+ * behind an opaque handle, for callers without C++ (ctypes, cgo). Fields are
+ * set by name: "qubits", "exec_type", "gpu_count", "block_size",
+ * "repeat_count", "gpu_region_bits", "gpu_coalesced_bits", "ops_block",
+ * "planner", "kernel", "thread_count", "cpu_region_bits", "cpu_coalesced_bits",
+ * "resident". */
+typedef struct hgm_dgm hgm_dgm;
+int hgm_dgm_create(int dtype, hgm_dgm **out);
+void hgm_dgm_destroy(hgm_dgm *d);
+int hgm_dgm_set(hgm_dgm *d, const char *field, int64_t value);
+int hgm_dgm_get(const hgm_dgm *d, const char *field, int64_t *value);
+int hgm_dgm_allocate_memory(hgm_dgm *d);                       /* dgm_core.cpp:54-56 */
+int hgm_dgm_free_memory(hgm_dgm *d);
+int hgm_dgm_set_memory_value(hgm_dgm *d, int64_t pos);         /* dgm_core.cpp:63-65 */
+void *hgm_dgm_state(hgm_dgm *d);                               /* host amplitudes */
+int hgm_dgm_add_gate(hgm_dgm *d, const char *name, const double *m8, int *added); /* gates.cpp:42-60 */
+int hgm_dgm_get_gate(const hgm_dgm *d, const char *name, double *m8, int *found);
+int hgm_dgm_set_function(hgm_dgm *d, const char *function, int iterations, int reset); /* dgm_parser.cpp:29-35 */
+int hgm_dgm_set_function_steps(hgm_dgm *d, const char *const *steps, int n_steps, int iterations, int reset);
+int hgm_dgm_num_pts(const hgm_dgm *d, int64_t *n);
+int hgm_dgm_get_ops(const hgm_dgm *d, hgm_op *out, int64_t cap);
+int hgm_dgm_execute(hgm_dgm *d, int iterations);               /* dgm_core.cpp:243-267 */
+int hgm_dgm_execute_function(hgm_dgm *d, const char *function, int iterations);
+int hgm_dgm_measure(hgm_dgm *d, int qubit, int *outcome);      /* dgm_core.cpp:69-125 */
+/* Marginal distribution over the given qubits: probs[v] for each value v of
+ * the masked index bits, listed in ascending mask-value order (dgm_core.cpp:150-162). */
+int hgm_dgm_measure_many(hgm_dgm *d, const int *qubits, int n, int64_t *values, double *probs, int64_t *n_out);
+int hgm_dgm_sync_host(hgm_dgm *d);                             /* resident mode: download */
+int hgm_dgm_stats(const hgm_dgm *d, hgm_stats *out);
+int hgm_dgm_count_ops(hgm_dgm *d, int iterations, int64_t counts[7]); /* CountOps, dgm_core.cpp:270-299 */
+void hgm_seed_rng(uint64_t seed);                              /* seed_rng, common.cpp:12-14 */
+
+/* Circuit step generators and builders by name (src/core/gates.cpp:69-163,
+ * src/algorithms/lib_grover.cpp, src/algorithms/lib_shor_circuits.cpp, plus
+ * the benchmark circuits). Builders that register gates do so in d's catalog.
+ * Multi-step results are joined with ';'. Returns HGM_OK and sets *len to the
+ * text length; out may be NULL to query the length. */
+int hgm_generate(hgm_dgm *d, const char *which, const int64_t *args, int n_args, const char *op,
+                 char *out, size_t cap, size_t *len);
+
+/* ------------------------------------------------------------------ reference ABI
+ * Memory layout of the reference's struct PT (include/core/common.h:57-90) on
+ * LP64, so that a reference build can hand its PT** straight to this library. */
+typedef struct hgm_ref_pt {
+    int qubits;
+    hgm_cfloat *matrix;
+    int matrix_size;
+    int span_start_bit, target_bit;
+    bool affected;
+    long control_value, control_mask;
+    long *control_bit_positions, control_count;
+    long *control_rest, control_rest_count;
+} hgm_ref_pt;
+
+/* replaces setDevice (src/core/kernel.cu:31-33; include/core/dgm.h:22).
+ * Like the reference it returns the runtime's error flag: false on success. */
+bool setDevice(int device_id);
+
+/* replaces GpuExecutionWrapper (src/core/kernel.cu:303-316; include/core/dgm.h:30).
+ * Same argument order. Runs the NULL-terminated PT list on the FP32 host state
+ * in place and returns it. gpu_count must be 1. OPS_BLOCK comes from the
+ * HGM_OPS_BLOCK environment variable (default 300). Errors print and exit(1). */
+hgm_cfloat *GpuExecutionWrapper(hgm_cfloat *read_memory, hgm_ref_pt **pts, int qubits, int coalesced_bits,
+                                int gpu_region_bits, int gpu_count, int block_size, int repeat_count,
+                                int iterations);
+
+/* replace ProjectState / GetState (src/core/kernel.cu:321,384; include/core/dgm.h:31-32).
+ * t_HYBRID is outside this library: both print a message and return false,
+ * as src/core/kernel_stub.cpp:27-33 does. */
+bool ProjectState(hgm_cfloat *state, int qubits, int region_size, long region_id, long region_mask, int gpu_count);
+bool GetState(hgm_cfloat *state, int qubits, int region_size, long region_id, long region_mask, int gpu_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HGM_GPU_H */

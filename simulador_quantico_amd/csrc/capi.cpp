@@ -1,0 +1,356 @@
+// extern "C" layer over the executor: the DGM handle, the circuit generators
+// by name, and the reference-compatible entry points of include/core/dgm.h:22-32.
+#include <chrono>
+#include <cstddef>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+
+#include "hgm_dgm.hpp"
+#include "hgm_gpu.h"
+#include "runtime.hpp"
+
+static_assert(offsetof(hgm_ref_pt, matrix) == 8, "PT layout (include/core/common.h:57-90)");
+static_assert(offsetof(hgm_ref_pt, target_bit) == 24, "PT layout");
+static_assert(offsetof(hgm_ref_pt, control_value) == 32, "PT layout");
+static_assert(offsetof(hgm_ref_pt, control_mask) == 40, "PT layout");
+static_assert(offsetof(hgm_ref_pt, control_count) == 56, "PT layout");
+static_assert(sizeof(hgm_ref_pt) == 80, "PT layout");
+static_assert(sizeof(hgm_op) == 88, "hgm_op layout");
+
+struct hgm_dgm {
+    int dtype;
+    std::unique_ptr<hgm::DGM> d64;
+    std::unique_ptr<hgm::DGM32> d32;
+    hgm::DGMCore &core() { return d64 ? static_cast<hgm::DGMCore &>(*d64) : static_cast<hgm::DGMCore &>(*d32); }
+    const hgm::DGMCore &core() const {
+        return d64 ? static_cast<const hgm::DGMCore &>(*d64) : static_cast<const hgm::DGMCore &>(*d32);
+    }
+};
+
+namespace {
+
+template <class F>
+int guard(F fn) {
+    try {
+        return fn();
+    } catch (const hgm::Error &e) {
+        hgm::set_error(e.what());
+        return e.code ? e.code : HGM_E_ARG;
+    } catch (const std::exception &e) {
+        hgm::set_error(e.what());
+        return HGM_E_ARG;
+    }
+}
+
+template <class F>
+auto visit(hgm_dgm *d, F fn) {
+    return d->d64 ? fn(*d->d64) : fn(*d->d32);
+}
+
+long *field(hgm::DGMCore &c, const std::string &f) {
+    static long scratch;
+    if (f == "qubits") return &c.qubits;
+    if (f == "thread_count") return &c.thread_count;
+    if (f == "cpu_region_bits") return &c.cpu_region_bits;
+    if (f == "cpu_coalesced_bits") return &c.cpu_coalesced_bits;
+    if (f == "gpu_region_bits") return &c.gpu_region_bits;
+    if (f == "gpu_coalesced_bits") return &c.gpu_coalesced_bits;
+    (void)scratch;
+    return nullptr;
+}
+
+int *ifield(hgm::DGMCore &c, const std::string &f) {
+    if (f == "exec_type") return &c.exec_type;
+    if (f == "gpu_count") return &c.gpu_count;
+    if (f == "block_size") return &c.block_size;
+    if (f == "repeat_count") return &c.repeat_count;
+    if (f == "ops_block") return &c.ops_block;
+    if (f == "planner") return &c.planner;
+    if (f == "kernel") return &c.kernel;
+    return nullptr;
+}
+
+std::string join(const std::vector<std::string> &v) {
+    std::string r;
+    for (size_t i = 0; i < v.size(); ++i) { if (i) r += ";"; r += v[i]; }
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hgm_dgm_create(int dtype, hgm_dgm **out) {
+    if (!out || (dtype != HGM_C128 && dtype != HGM_C64)) { hgm::set_error("bad dtype"); return HGM_E_ARG; }
+    hgm_dgm *h = new hgm_dgm();
+    h->dtype = dtype;
+    if (dtype == HGM_C128) h->d64.reset(new hgm::DGM());
+    else h->d32.reset(new hgm::DGM32());
+    h->core().exit_on_error = false;
+    *out = h;
+    return HGM_OK;
+}
+
+void hgm_dgm_destroy(hgm_dgm *d) { delete d; }
+
+int hgm_dgm_set(hgm_dgm *d, const char *name, int64_t value) {
+    if (!d || !name) return HGM_E_ARG;
+    hgm::DGMCore &c = d->core();
+    const std::string f(name);
+    if (long *p = field(c, f)) { *p = (long)value; return HGM_OK; }
+    if (int *p = ifield(c, f)) { *p = (int)value; return HGM_OK; }
+    if (f == "resident") { c.resident = value != 0; return HGM_OK; }
+    hgm::set_error("unknown DGM field '" + f + "'");
+    return HGM_E_ARG;
+}
+
+int hgm_dgm_get(const hgm_dgm *d, const char *name, int64_t *value) {
+    if (!d || !name || !value) return HGM_E_ARG;
+    hgm::DGMCore &c = const_cast<hgm_dgm *>(d)->core();
+    const std::string f(name);
+    if (long *p = field(c, f)) { *value = *p; return HGM_OK; }
+    if (int *p = ifield(c, f)) { *value = *p; return HGM_OK; }
+    if (f == "resident") { *value = c.resident; return HGM_OK; }
+    hgm::set_error("unknown DGM field '" + f + "'");
+    return HGM_E_ARG;
+}
+
+int hgm_dgm_allocate_memory(hgm_dgm *d) {
+    return guard([&] { visit(d, [](auto &x) { x.allocateMemory(); return 0; }); return HGM_OK; });
+}
+
+int hgm_dgm_free_memory(hgm_dgm *d) {
+    return guard([&] { visit(d, [](auto &x) { x.freeMemory(); return 0; }); return HGM_OK; });
+}
+
+int hgm_dgm_set_memory_value(hgm_dgm *d, int64_t pos) {
+    return guard([&] {
+        return visit(d, [&](auto &x) {
+            if (!x.state || pos < 0 || pos >= (int64_t(1) << x.qubits)) {
+                hgm::set_error("setMemoryValue: position outside the state");
+                return (int)HGM_E_ARG;
+            }
+            x.setMemoryValue((long)pos);
+            return (int)HGM_OK;
+        });
+    });
+}
+
+void *hgm_dgm_state(hgm_dgm *d) {
+    return d ? visit(d, [](auto &x) { return static_cast<void *>(x.state); }) : nullptr;
+}
+
+int hgm_dgm_add_gate(hgm_dgm *d, const char *name, const double *m8, int *added) {
+    if (!d || !name || !m8) return HGM_E_ARG;
+    hgm::cplx m[4] = {{m8[0], m8[1]}, {m8[2], m8[3]}, {m8[4], m8[5]}, {m8[6], m8[7]}};
+    bool ok = d->core().gates.addGate(name, m);
+    if (added) *added = ok;
+    return HGM_OK;
+}
+
+int hgm_dgm_get_gate(const hgm_dgm *d, const char *name, double *m8, int *found) {
+    if (!d || !name || !m8) return HGM_E_ARG;
+    const hgm::cplx *m = d->core().gates.getMatrix(name);
+    if (found) *found = m != nullptr;
+    if (m)
+        for (int k = 0; k < 4; ++k) { m8[2 * k] = m[k].real(); m8[2 * k + 1] = m[k].imag(); }
+    return HGM_OK;
+}
+
+int hgm_dgm_set_function(hgm_dgm *d, const char *function, int iterations, int reset) {
+    if (!d || !function) return HGM_E_ARG;
+    return guard([&] { d->core().setFunction(std::string(function), iterations, reset != 0); return HGM_OK; });
+}
+
+int hgm_dgm_set_function_steps(hgm_dgm *d, const char *const *steps, int n, int iterations, int reset) {
+    if (!d || (n > 0 && !steps)) return HGM_E_ARG;
+    std::vector<std::string> v;
+    for (int i = 0; i < n; ++i) v.emplace_back(steps[i]);
+    return guard([&] { d->core().setFunction(v, iterations, reset != 0); return HGM_OK; });
+}
+
+int hgm_dgm_num_pts(const hgm_dgm *d, int64_t *n) {
+    if (!d || !n) return HGM_E_ARG;
+    const hgm::DGMCore &c = d->core();
+    int64_t k = 0;
+    while (c.pts && c.pts[k]) ++k;
+    *n = k;
+    return HGM_OK;
+}
+
+int hgm_dgm_get_ops(const hgm_dgm *d, hgm_op *out, int64_t cap) {
+    if (!d || (!out && cap)) return HGM_E_ARG;
+    std::vector<hgm_op> o = d->core().ops();
+    if ((int64_t)o.size() > cap) { hgm::set_error("op buffer too small"); return HGM_E_ARG; }
+    std::memcpy(out, o.data(), o.size() * sizeof(hgm_op));
+    return HGM_OK;
+}
+
+int hgm_dgm_execute(hgm_dgm *d, int iterations) {
+    if (!d) return HGM_E_ARG;
+    return guard([&] { visit(d, [&](auto &x) { x.execute(iterations); return 0; }); return HGM_OK; });
+}
+
+int hgm_dgm_execute_function(hgm_dgm *d, const char *function, int iterations) {
+    if (!d || !function) return HGM_E_ARG;
+    return guard([&] {
+        visit(d, [&](auto &x) { x.executeFunction(std::string(function), iterations); return 0; });
+        return HGM_OK;
+    });
+}
+
+int hgm_dgm_measure(hgm_dgm *d, int qubit, int *outcome) {
+    if (!d || !outcome) return HGM_E_ARG;
+    return guard([&] { *outcome = visit(d, [&](auto &x) { return x.measure(qubit); }); return HGM_OK; });
+}
+
+int hgm_dgm_measure_many(hgm_dgm *d, const int *qubits, int n, int64_t *values, double *probs, int64_t *n_out) {
+    if (!d || (!qubits && n) || !n_out) return HGM_E_ARG;
+    return guard([&] {
+        std::vector<int> q(qubits, qubits + n);
+        std::map<long, double> m = visit(d, [&](auto &x) { return x.measure(q); });
+        int64_t cap = *n_out, k = 0;
+        for (auto &kv : m) {
+            if (k < cap && values && probs) { values[k] = kv.first; probs[k] = kv.second; }
+            ++k;
+        }
+        *n_out = k;
+        return HGM_OK;
+    });
+}
+
+int hgm_dgm_sync_host(hgm_dgm *d) {
+    if (!d) return HGM_E_ARG;
+    return guard([&] { visit(d, [](auto &x) { x.syncHost(); return 0; }); return HGM_OK; });
+}
+
+int hgm_dgm_stats(const hgm_dgm *d, hgm_stats *out) {
+    if (!d || !out) return HGM_E_ARG;
+    *out = d->core().last_stats;
+    return HGM_OK;
+}
+
+int hgm_dgm_count_ops(hgm_dgm *d, int iterations, int64_t counts[7]) {
+    if (!d || !counts) return HGM_E_ARG;
+    hgm::DGMCore &c = d->core();
+    c.CountOps(iterations);
+    counts[0] = c.total_op; counts[1] = c.dense; counts[2] = c.main_diag; counts[3] = c.sec_diag;
+    counts[4] = c.c_dense; counts[5] = c.c_main_diag; counts[6] = c.c_sec_diag;
+    return HGM_OK;
+}
+
+void hgm_seed_rng(uint64_t seed) { hgm::seed_rng((unsigned long)seed); }
+
+int hgm_generate(hgm_dgm *d, const char *which, const int64_t *a, int na, const char *op, char *out, size_t cap,
+                 size_t *len) {
+    if (!which || !len || (na > 0 && !a)) return HGM_E_ARG;
+    return guard([&]() -> int {
+        const std::string w(which), o(op ? op : "");
+        auto need = [&](int k) {
+            if (na < k) throw hgm::Error(HGM_E_ARG, w + " needs " + std::to_string(k) + " integer arguments");
+        };
+        auto gates = [&]() -> hgm::Gates & {
+            if (!d) throw hgm::Error(HGM_E_ARG, w + " registers gates and needs a DGM handle");
+            return d->core().gates;
+        };
+        std::string s;
+        if (w == "CNot") { need(4); s = hgm::CNot(a[0], a[1], a[2], a[3]); }
+        else if (w == "Toffoli") { need(5); s = hgm::Toffoli(a[0], a[1], a[2], a[3], a[4]); }
+        else if (w == "Controlled1") { need(4); s = hgm::Controlled1(a[0], a[1], a[2], o, a[3]); }
+        else if (w == "Controlled2") { need(5); s = hgm::Controlled2(a[0], a[1], a[2], a[3], o, a[4]); }
+        else if (w == "ControlledN") {   // qubits, target, control_value, ctrls...
+            need(3);
+            std::vector<int> ctrls(a + 3, a + na);
+            s = hgm::ControlledN(a[0], ctrls, a[1], o, a[2]);
+        }
+        else if (w == "Pauli_X") { need(3); s = hgm::Pauli_X(a[0], a[1], a[2]); }
+        else if (w == "Pauli_Z") { need(3); s = hgm::Pauli_Z(a[0], a[1], a[2]); }
+        else if (w == "Hadamard") { need(3); s = hgm::Hadamard(a[0], a[1], a[2]); }
+        else if (w == "Oracle1") { need(2); s = hgm::Oracle1(a[0], a[1]); }
+        else if (w == "ControledZ") { need(1); s = hgm::ControledZ(a[0]); }
+        else if (w == "QFT") { need(4); s = join(hgm::QFT(a[0], a[1], a[2], a[3], gates())); }
+        else if (w == "RQFT") { need(4); s = join(hgm::RQFT(a[0], a[1], a[2], a[3], gates())); }
+        else if (w == "AddF") { need(5); s = hgm::AddF(a[0], a[1], a[2], a[3], a[4], gates()); }
+        else if (w == "SubF") { need(5); s = hgm::SubF(a[0], a[1], a[2], a[3], a[4], gates()); }
+        else if (w == "AddFc") { need(6); s = join(hgm::AddF(a[0], a[1], a[2], a[3], a[4], a[5] != 0, gates())); }
+        else if (w == "SubFc") { need(6); s = join(hgm::SubF(a[0], a[1], a[2], a[3], a[4], a[5] != 0, gates())); }
+        else if (w == "CAddF") { need(6); s = hgm::CAddF(a[0], a[1], a[2], a[3], a[4], a[5], gates()); }
+        else if (w == "C2AddF") { need(7); s = hgm::C2AddF(a[0], a[1], a[2], a[3], a[4], a[5], a[6], gates()); }
+        else if (w == "CSubF") { need(6); s = hgm::CSubF(a[0], a[1], a[2], a[3], a[4], a[5], gates()); }
+        else if (w == "C2SubF") { need(7); s = hgm::C2SubF(a[0], a[1], a[2], a[3], a[4], a[5], a[6], gates()); }
+        else if (w == "genRot") { need(3); s = hgm::genRot(a[0], a[1], a[2], gates()); }
+        else if (w == "CSwapR") { need(5); s = join(hgm::CSwapR(a[0], a[1], a[2], a[3], a[4])); }
+        else if (w == "SwapOver") { need(3); s = join(hgm::SwapOver(a[0], a[1], a[2])); }
+        else if (w == "GroverIterationRef") { need(2); s = join(hgm::GroverIterationRef(a[0], a[1])); }
+        else if (w == "GroverIteration") { need(2); s = join(hgm::GroverIteration(a[0], (uint64_t)a[1])); }
+        else if (w == "GroverIterations") { need(1); s = std::to_string(hgm::GroverIterations(a[0])); }
+        else if (w == "RandomLayers") { need(3); s = join(hgm::RandomLayers(a[0], a[1], (uint64_t)a[2], gates())); }
+        else if (w == "DraperAdder") { need(1); s = join(hgm::DraperAdder(a[0], gates())); }
+        else throw hgm::Error(HGM_E_ARG, "unknown generator '" + w + "'");
+        *len = s.size();
+        if (out && cap > s.size()) std::memcpy(out, s.c_str(), s.size() + 1);
+        return HGM_OK;
+    });
+}
+
+// ------------------------------------------------------------------ reference ABI
+
+bool setDevice(int device_id) { return hgm_init(device_id) != HGM_OK; }
+
+hgm_cfloat *GpuExecutionWrapper(hgm_cfloat *read_memory, hgm_ref_pt **pts, int qubits, int coalesced_bits,
+                                int gpu_region_bits, int gpu_count, int block_size, int repeat_count,
+                                int iterations) {
+    static const bool debug = std::getenv("HYBRID_DEBUG") != nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    if (gpu_count != 1) {
+        std::printf("error: gpu_count %d: this library drives one GPU per process\n", gpu_count);
+        std::exit(1);
+    }
+    std::vector<hgm_op> ops;
+    for (size_t i = 0; pts && pts[i]; ++i) {
+        const hgm_ref_pt *p = pts[i];
+        hgm_op o;
+        std::memset(&o, 0, sizeof(o));
+        o.target_bit = p->target_bit;
+        o.control_mask = (uint64_t)p->control_mask;
+        o.control_value = (uint64_t)p->control_value;
+        for (int k = 0; k < 4; ++k) {
+            o.m[2 * k] = __real__ p->matrix[k];
+            o.m[2 * k + 1] = __imag__ p->matrix[k];
+        }
+        ops.push_back(o);
+    }
+    hgm_tuning t;
+    hgm_default_tuning(&t);
+    t.block_size = block_size;
+    t.repeat_count = repeat_count;
+    t.gpu_region_bits = gpu_region_bits;
+    t.gpu_coalesced_bits = coalesced_bits;
+    if (const char *e = std::getenv("HGM_OPS_BLOCK")) t.ops_block = std::atoi(e);
+    hgm_stats st;
+    std::memset(&st, 0, sizeof(st));
+    int rc = hgm_execute(read_memory, qubits, HGM_C64, ops.data(), (int64_t)ops.size(), &t, iterations, &st);
+    if (rc) {
+        std::printf("error: %d - %s\n", rc, hgm_last_error());
+        std::exit(1);
+    }
+    if (debug)
+        std::fprintf(stderr, "[GPUEXEC]   qubits=%d elapsed_ms=%.3f passes=%lld kernel_ms=%.3f\n", qubits,
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(),
+                     (long long)st.passes, st.kernel_ms);
+    return read_memory;
+}
+
+bool ProjectState(hgm_cfloat *, int, int, long, long, int) {
+    std::fprintf(stderr, "hgm: ProjectState belongs to t_HYBRID, which this library does not provide.\n");
+    return false;
+}
+
+bool GetState(hgm_cfloat *, int, int, long, long, int) {
+    std::fprintf(stderr, "hgm: GetState belongs to t_HYBRID, which this library does not provide.\n");
+    return false;
+}
+
+}  // extern "C"

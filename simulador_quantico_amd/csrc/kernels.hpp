@@ -1,0 +1,39 @@
+// Launch wrappers of the gfx950 kernels (defined in kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "planner.hpp"
+
+namespace hgm {
+
+// Per-pass launch arguments shared by the pass kernels.
+struct PassArgs {
+    uint64_t nonlocal_mask;            // state-index bits that number the tiles
+    int64_t n_tiles;                   // 2^(qubits - r)
+    int32_t r, c, B, R;                // tile shape
+    int8_t local_pos[kMaxTileBits];    // tile bit j -> state-index bit
+};
+
+// One full-state pass of the LDS tile kernel: every tile is loaded into LDS,
+// the pass's gates are applied there, and the tile is written back.
+hipError_t launch_pass_lds(int dtype, void *state, const DevOp *ops, int n_ops, const PassArgs &a, hipStream_t s);
+
+// state[i] = (i == index) for i < n_amps.
+hipError_t launch_set_basis(int dtype, void *state, uint64_t n_amps, uint64_t index, hipStream_t s);
+
+// Per-block FP64 partial sums of |a|^2 split by index bit `bit`
+// (bit < 0: everything lands in the first sum). partial holds 2*blocks doubles.
+hipError_t launch_prob_halves(int dtype, const void *state, uint64_t n_amps, int bit, double *partial, int blocks,
+                              hipStream_t s);
+
+// Per-block maximum of |a|^2 with the lowest index among ties.
+hipError_t launch_argmax(int dtype, const void *state, uint64_t n_amps, double *pmax, uint64_t *pidx, int blocks,
+                         hipStream_t s);
+
+// Collapse `bit` to `outcome`, moving the kept half to the bit=0 slot divided
+// by norm and zeroing the bit=1 slot (src/core/dgm_core.cpp:115-122).
+hipError_t launch_collapse(int dtype, void *state, uint64_t n_amps, int bit, int outcome, double norm, hipStream_t s);
+
+}  // namespace hgm

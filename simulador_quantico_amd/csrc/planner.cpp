@@ -1,0 +1,263 @@
+// Pass planner (see planner.hpp).
+#include "planner.hpp"
+
+#include <chrono>
+#include <cstring>
+
+namespace hgm {
+
+namespace {
+
+constexpr int kMaxQubits = 40;
+constexpr size_t kMaxLdsBytes = 160 * 1024;   // gfx950 LDS per workgroup
+constexpr size_t kLdsReserve = 2 * 1024;      // bit-deposit tables of the LDS kernel
+
+bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+int log2i(int64_t v) { int k = 0; while ((int64_t(1) << k) < v) ++k; return k; }
+
+struct GateInfo {
+    int kind;
+    int target;
+    uint64_t need;    // bits that must be tile-local (target of a non-diagonal gate)
+    uint64_t writes;  // bits whose amplitudes the gate mixes
+    uint64_t reads;   // bits the gate only inspects (controls, diagonal targets)
+};
+
+void fill_pass_ops(const std::vector<GateInfo> &info, const hgm_op *ops, int64_t n_ops,
+                   const std::vector<int64_t> &chosen, Pass &pass, Plan &plan) {
+    int tile_of[64];
+    for (int b = 0; b < 64; ++b) tile_of[b] = -1;
+    for (int j = 0; j < plan.shape.r; ++j) tile_of[(int)pass.local_pos[j]] = j;
+    pass.op_begin = (int64_t)plan.ops.size();
+    for (int64_t gi : chosen) {
+        const hgm_op &o = ops[gi % n_ops];
+        const GateInfo &g = info[gi % n_ops];
+        DevOp d;
+        std::memset(&d, 0, sizeof(d));
+        std::memcpy(d.m, o.m, sizeof(d.m));
+        if (g.kind == K_DIAG && tile_of[g.target] < 0) {
+            d.kind = K_DIAG_GLOBAL;
+            d.tbit = g.target;
+        } else {
+            d.kind = g.kind;
+            d.tbit = tile_of[g.target];
+        }
+        for (int b = 0; b < 64; ++b) {
+            if (!((o.control_mask >> b) & 1)) continue;
+            uint64_t v = (o.control_value >> b) & 1;
+            if (tile_of[b] >= 0) {
+                d.lmask |= 1u << tile_of[b];
+                d.lval |= uint32_t(v) << tile_of[b];
+            } else {
+                d.gmask |= uint64_t(1) << b;
+                d.gval |= v << b;
+            }
+        }
+        plan.ops.push_back(d);
+        plan.src_index.push_back(gi);
+    }
+    pass.op_count = (int64_t)plan.ops.size() - pass.op_begin;
+}
+
+void finish_pass(uint64_t local, const Plan &plan, Pass &pass) {
+    const int n = plan.qubits, c = plan.shape.c, r = plan.shape.r;
+    // top up to r local bits with the lowest free bits
+    int have = __builtin_popcountll(local);
+    for (int b = 0; b < n && have < r; ++b)
+        if (!((local >> b) & 1)) { local |= uint64_t(1) << b; ++have; }
+    pass.local_mask = local;
+    std::memset(pass.local_pos, -1, sizeof(pass.local_pos));
+    int j = 0;
+    for (int b = 0; b < c; ++b) pass.local_pos[j++] = (int8_t)b;
+    for (int b = c; b < n; ++b)
+        if ((local >> b) & 1) pass.local_pos[j++] = (int8_t)b;
+}
+
+}  // namespace
+
+int classify(const double m[8]) {
+    auto zero = [&](int k) { return m[2 * k] == 0.0 && m[2 * k + 1] == 0.0; };
+    if (zero(1) && zero(2)) return K_DIAG;
+    if (zero(0) && zero(3)) return K_ANTI;
+    return K_DENSE;
+}
+
+size_t amp_bytes(int dtype) { return dtype == HGM_C64 ? 8 : 16; }
+
+int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::string *err) {
+    if (qubits < 1 || qubits > kMaxQubits) {
+        *err = "qubits must be in [1, " + std::to_string(kMaxQubits) + "], got " + std::to_string(qubits);
+        return HGM_E_ARG;
+    }
+    if (dtype != HGM_C128 && dtype != HGM_C64) { *err = "unknown amplitude dtype"; return HGM_E_ARG; }
+    const int B = t.block_size, R = t.repeat_count, r = t.gpu_region_bits, c = t.gpu_coalesced_bits;
+    if (B < 1 || R < 1 || r < 1 || r > kMaxTileBits) {
+        *err = "Erro de tuning: block_size, repeat_count and gpu_region_bits must be positive (gpu_region_bits <= " +
+               std::to_string(kMaxTileBits) + ")";
+        return HGM_E_TUNING;
+    }
+    // src/core/dgm_core.cpp:207-212
+    if (2LL * B * R != (int64_t(1) << r)) {
+        *err = "Erro de tuning: 2*block_size*repeat_count (" + std::to_string(2LL * B * R) +
+               ") precisa ser igual a 2^gpu_region_bits (" + std::to_string(int64_t(1) << r) + ")";
+        return HGM_E_TUNING;
+    }
+    // src/core/dgm_core.cpp:218-223
+    if (c < 0 || r < c) {
+        *err = "Erro de tuning: gpu_region_bits (" + std::to_string(r) + ") precisa ser >= gpu_coalesced_bits (" +
+               std::to_string(c) + ")";
+        return HGM_E_TUNING;
+    }
+    if (B > 1024) { *err = "Erro de tuning: block_size must be <= 1024 on gfx950"; return HGM_E_TUNING; }
+    if (!is_pow2(B) || !is_pow2(R)) { *err = "Erro de tuning: block_size and repeat_count must be powers of two"; return HGM_E_TUNING; }
+    if (t.ops_block < 1) { *err = "Erro de tuning: ops_block must be >= 1"; return HGM_E_TUNING; }
+    if (t.planner != HGM_PLAN_GREEDY && t.planner != HGM_PLAN_WINDOW) { *err = "unknown planner"; return HGM_E_TUNING; }
+    if (t.kernel < HGM_KERNEL_AUTO || t.kernel > HGM_KERNEL_REG) { *err = "unknown kernel family"; return HGM_E_TUNING; }
+    if ((size_t(1) << r) * amp_bytes(dtype) > kMaxLdsBytes - kLdsReserve) {
+        *err = "Erro de tuning: a tile of 2^" + std::to_string(r) + " amplitudes exceeds the 160 KiB LDS of a gfx950 workgroup";
+        return HGM_E_TUNING;
+    }
+    TileShape s;
+    if (qubits <= r) {
+        // the whole state is one tile; shrink the workgroup to the state
+        s.r = qubits;
+        s.c = c < qubits ? c : qubits;
+        int64_t pairs = int64_t(1) << (qubits - 1);
+        s.B = (int)(B < pairs ? B : pairs);
+        s.R = (int)(pairs / s.B);
+    } else {
+        // a pass must have at least one selectable bit beyond the coalesced
+        // ones, or a gate on a high bit could never be scheduled (the
+        // reference loops forever there, src/core/kernel.cu:208,281)
+        if (c >= r) {
+            *err = "Erro de tuning: gpu_region_bits (" + std::to_string(r) + ") must exceed gpu_coalesced_bits (" +
+                   std::to_string(c) + ") when the state has more qubits than a tile";
+            return HGM_E_TUNING;
+        }
+        s.r = r; s.c = c; s.B = B; s.R = R;
+    }
+    *out = s;
+    return HGM_OK;
+}
+
+int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, const hgm_tuning &t,
+               Plan *out, std::string *err) {
+    auto t0 = std::chrono::steady_clock::now();
+    Plan plan;
+    plan.qubits = qubits;
+    plan.dtype = dtype;
+    int st = make_shape(qubits, dtype, t, &plan.shape, err);
+    if (st) return st;
+    if (iterations < 0) { *err = "iterations must be >= 0"; return HGM_E_ARG; }
+    if (n_ops < 0 || (n_ops > 0 && !ops)) { *err = "bad op array"; return HGM_E_ARG; }
+    plan.kernel = t.kernel == HGM_KERNEL_AUTO ? HGM_KERNEL_LDS : t.kernel;
+
+    const uint64_t all = qubits >= 64 ? ~uint64_t(0) : ((uint64_t(1) << qubits) - 1);
+    std::vector<GateInfo> info((size_t)n_ops);
+    for (int64_t i = 0; i < n_ops; ++i) {
+        const hgm_op &o = ops[i];
+        if (o.target_bit < 0 || o.target_bit >= qubits) {
+            *err = "gate " + std::to_string(i) + ": target bit " + std::to_string(o.target_bit) + " outside the state";
+            return HGM_E_ARG;
+        }
+        uint64_t tb = uint64_t(1) << o.target_bit;
+        if ((o.control_mask & ~all) || (o.control_mask & tb) || (o.control_value & ~o.control_mask)) {
+            *err = "gate " + std::to_string(i) + ": control mask/value inconsistent with the target or the state";
+            return HGM_E_ARG;
+        }
+        GateInfo g;
+        g.kind = classify(o.m);
+        g.target = o.target_bit;
+        if (g.kind == K_DIAG) { g.need = 0; g.writes = 0; g.reads = tb | o.control_mask; }
+        else { g.need = tb; g.writes = tb; g.reads = o.control_mask; }
+        info[i] = g;
+    }
+
+    const int64_t N = n_ops * (int64_t)iterations;   // expanded list: gate k is ops[k % n_ops]
+    plan.gates = N;
+    const int r = plan.shape.r, c = plan.shape.c;
+    const uint64_t low = c >= 64 ? ~uint64_t(0) : ((uint64_t(1) << c) - 1);
+    const int64_t window = t.lookahead > 0 ? t.lookahead : 4096;
+    const int64_t cap = t.ops_block;
+
+    std::vector<int64_t> chosen;
+    if (qubits <= r) {
+        // whole state in one tile: only the ops_block cap splits passes
+        for (int64_t k = 0; k < N; k += cap) {
+            chosen.clear();
+            for (int64_t j = k; j < N && j < k + cap; ++j) chosen.push_back(j);
+            Pass p;
+            finish_pass(all, plan, p);
+            fill_pass_ops(info, ops, n_ops, chosen, p, plan);
+            plan.passes.push_back(p);
+        }
+    } else if (t.planner == HGM_PLAN_WINDOW) {
+        // In order; targets >= c must fit one window of e = r-c contiguous bits.
+        const int e = r - c;
+        int64_t k = 0;
+        while (k < N) {
+            chosen.clear();
+            bool have = false;
+            int lo = 0, hi = 0;
+            while (k < N && (int64_t)chosen.size() < cap) {
+                const GateInfo &g = info[k % n_ops];
+                if (g.need && g.target >= c) {
+                    int b = g.target;
+                    if (!have) { lo = hi = b; have = true; }
+                    else if (b >= lo && b - lo < e) { if (b > hi) hi = b; }
+                    else if (b <= hi && hi - b < e) { if (b < lo) lo = b; }
+                    else break;
+                }
+                chosen.push_back(k++);
+            }
+            int rs = have ? (hi - e + 1 > c ? hi - e + 1 : c) : c;
+            if (rs + e > qubits) rs = qubits - e;
+            uint64_t local = low | (((uint64_t(1) << e) - 1) << rs);
+            Pass p;
+            finish_pass(local, plan, p);
+            fill_pass_ops(info, ops, n_ops, chosen, p, plan);
+            plan.passes.push_back(p);
+        }
+    } else {
+        // Greedy: scan ahead from the oldest unscheduled gate; take every gate
+        // that commutes with all the gates skipped before it and whose target
+        // is local or can still be made local.
+        std::vector<char> done((size_t)N, 0);
+        int64_t head = 0;
+        while (head < N) {
+            chosen.clear();
+            uint64_t local = low;
+            int nlocal = c;
+            uint64_t bw = 0, br = 0;   // writes / reads of the skipped gates
+            int64_t scanned = 0;
+            for (int64_t k = head; k < N && (int64_t)chosen.size() < cap && scanned < window; ++k) {
+                if (done[k]) continue;
+                ++scanned;
+                const GateInfo &g = info[k % n_ops];
+                bool ok = !(g.writes & (bw | br)) && !(g.reads & bw);
+                if (ok && (g.need & ~local)) {
+                    if (nlocal < r) { local |= g.need; ++nlocal; }
+                    else ok = false;
+                }
+                if (ok) {
+                    chosen.push_back(k);
+                    done[k] = 1;
+                } else {
+                    bw |= g.writes;
+                    br |= g.reads;
+                    if (bw == all) break;
+                }
+            }
+            while (head < N && done[head]) ++head;
+            Pass p;
+            finish_pass(local, plan, p);
+            fill_pass_ops(info, ops, n_ops, chosen, p, plan);
+            plan.passes.push_back(p);
+        }
+    }
+    plan.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = std::move(plan);
+    return HGM_OK;
+}
+
+}  // namespace hgm

@@ -1,0 +1,86 @@
+// Pass planner: turns a gate list into full-state passes over tiles.
+//
+// A pass streams the whole state once through tiles of 2^r amplitudes. The r
+// tile-local index bits are the low c bits (contiguous in memory, so loads
+// coalesce) plus r-c more bits chosen per pass; the remaining bits number the
+// tiles. Every gate scheduled into a pass must be executable inside one tile:
+//   * a non-diagonal gate needs its target bit local;
+//   * a diagonal gate needs nothing local (a non-local target just selects
+//     m00 or m11 for the whole tile);
+//   * controls may sit anywhere: local controls are tested per amplitude,
+//     non-local ones once per tile against the tile's base index.
+// This generalises the batching of the reference (src/core/kernel.cu:199-235),
+// which requires target bits in one contiguous window and forces a new pass
+// for any gate outside it, diagonal or not.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "hgm_gpu.h"
+
+namespace hgm {
+
+enum OpKind : int32_t {
+    K_DENSE = 0,        // full 2x2 update of the pair
+    K_DIAG = 1,         // m01 == m10 == 0, local target: per-amplitude phase
+    K_ANTI = 2,         // m00 == m11 == 0: swap-and-scale of the pair
+    K_DIAG_GLOBAL = 3   // diagonal gate on a non-local target: one phase for the tile
+};
+
+// Exact-zero classification of PT::matrixType (src/core/common.cpp:33-38).
+int classify(const double m[8]);
+
+// One gate in tile coordinates, as the pass kernels read it.
+struct DevOp {
+    int32_t kind;
+    int32_t tbit;       // tile bit of the target, or its state-index bit for K_DIAG_GLOBAL
+    uint32_t lmask;     // controls on tile-local bits (tile coordinates)
+    uint32_t lval;
+    uint64_t gmask;     // controls on non-local bits (state-index coordinates)
+    uint64_t gval;
+    double m[8];
+};
+static_assert(sizeof(DevOp) == 96, "DevOp layout is shared with the kernels");
+
+constexpr int kMaxTileBits = 16;
+
+struct TileShape {
+    int r;      // tile bits
+    int B;      // threads per tile
+    int R;      // amplitude pairs per thread
+    int c;      // coalesced (always-local) low bits
+};
+
+struct Pass {
+    uint64_t local_mask;              // state-index bits that are tile-local
+    int8_t local_pos[kMaxTileBits];   // tile bit j -> state-index bit
+    int64_t op_begin;                 // first DevOp of the pass
+    int64_t op_count;
+};
+
+struct Plan {
+    int qubits = 0;
+    int dtype = HGM_C128;
+    int kernel = HGM_KERNEL_LDS;
+    TileShape shape{};
+    std::vector<Pass> passes;
+    std::vector<DevOp> ops;
+    std::vector<int64_t> src_index;   // index of each DevOp in the iteration-expanded gate list
+    int64_t gates = 0;
+    double plan_ms = 0;
+};
+
+size_t amp_bytes(int dtype);
+
+// Validates a tuning tuple (the reference's rules of src/core/dgm_core.cpp:177-241
+// plus this library's limits) and derives the tile shape used for a state of
+// `qubits` qubits. Returns an HGM_* status and fills *err on failure.
+int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::string *err);
+
+// Builds the pass list for ops[0..n_ops) repeated `iterations` times.
+int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, const hgm_tuning &t,
+               Plan *out, std::string *err);
+
+}  // namespace hgm

@@ -1,0 +1,501 @@
+// HIP runtime of the t_GPU path: device context, device-resident states,
+// op-table upload, pass launches with HIP-event timing, device readout, and
+// the state / plan / one-shot part of the C ABI (include/hgm_gpu.h).
+//
+// Replaces the host orchestration of src/core/kernel.cu:131-291 (cudaMalloc +
+// H2D per call, __constant__ op upload and two device syncs per pass): here
+// all passes of a call are planned up front, their op tables go to the device
+// in one copy, and the passes are launched back to back on one stream with a
+// single synchronisation at the end.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hgm_gpu.h"
+#include "kernels.hpp"
+#include "planner.hpp"
+#include "runtime.hpp"
+
+struct hgm_state {
+    void *d = nullptr;
+    int qubits = 0;
+    int dtype = HGM_C128;
+    bool owned = true;
+    size_t bytes = 0;
+};
+
+struct hgm_plan {
+    hgm::Plan plan;
+};
+
+namespace hgm {
+
+namespace {
+
+thread_local std::string t_err;
+
+struct Ctx {
+    std::mutex mu;
+    bool ready = false;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr, e4 = nullptr, e5 = nullptr;
+    int cus = 0;
+    DevOp *ops = nullptr;       // op tables of the current call (grow-only)
+    size_t ops_cap = 0;
+    void *cache = nullptr;      // device buffer of hgm_execute (grow-only)
+    size_t cache_cap = 0;
+    double *partial = nullptr;  // reduction partials (device)
+    uint64_t *pidx = nullptr;
+    double *h_partial = nullptr;  // pinned host mirror
+    uint64_t *h_pidx = nullptr;
+};
+
+Ctx &ctx() {
+    static Ctx c;
+    return c;
+}
+
+constexpr int kRedBlocks = 1024;
+
+int hip_fail(hipError_t e, const char *what) {
+    t_err = std::string(what) + ": " + hipGetErrorString(e);
+    return HGM_E_HIP;
+}
+
+#define HGM_HIP(call, what)                            \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+int init_locked(int device) {
+    Ctx &c = ctx();
+    if (c.ready && (device < 0 || device == c.device)) return HGM_OK;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        t_err = "no HIP device is visible (the t_GPU path needs an MI355X)";
+        return HGM_E_HIP;
+    }
+    if (device < 0) device = 0;
+    if (device >= count) { t_err = "device index out of range"; return HGM_E_ARG; }
+    HGM_HIP(hipSetDevice(device), "hipSetDevice");
+    if (c.stream) {   // re-init on another device
+        hipStreamDestroy(c.stream);
+        c.stream = nullptr;
+    }
+    HGM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (hipEvent_t *e : {&c.e0, &c.e1, &c.e2, &c.e3, &c.e4, &c.e5}) {
+        if (!*e) HGM_HIP(hipEventCreate(e), "hipEventCreate");
+    }
+    hipDeviceProp_t prop;
+    HGM_HIP(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    c.cus = prop.multiProcessorCount;
+    if (!c.partial) {
+        HGM_HIP(hipMalloc(&c.partial, 2 * kRedBlocks * sizeof(double)), "hipMalloc(partials)");
+        HGM_HIP(hipMalloc(&c.pidx, kRedBlocks * sizeof(uint64_t)), "hipMalloc(partials)");
+        HGM_HIP(hipHostMalloc(&c.h_partial, 2 * kRedBlocks * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+        HGM_HIP(hipHostMalloc(&c.h_pidx, kRedBlocks * sizeof(uint64_t), hipHostMallocDefault), "hipHostMalloc");
+    }
+    c.device = device;
+    c.ready = true;
+    return HGM_OK;
+}
+
+int reserve_ops(size_t n) {
+    Ctx &c = ctx();
+    if (n <= c.ops_cap) return HGM_OK;
+    if (c.ops) { hipStreamSynchronize(c.stream); hipFree(c.ops); c.ops = nullptr; c.ops_cap = 0; }
+    size_t cap = n < 1024 ? 1024 : n + n / 2;
+    HGM_HIP(hipMalloc(&c.ops, cap * sizeof(DevOp)), "hipMalloc(op tables)");
+    c.ops_cap = cap;
+    return HGM_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+// Launches every pass of the plan on the context stream. Assumes the lock.
+int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st) {
+    Ctx &c = ctx();
+    if (s->qubits != p.qubits || s->dtype != p.dtype) {
+        t_err = "plan was built for another qubit count or dtype";
+        return HGM_E_STATE;
+    }
+    int rc = reserve_ops(p.ops.size());
+    if (rc) return rc;
+    if (!p.ops.empty())
+        HGM_HIP(hipMemcpyAsync(c.ops, p.ops.data(), p.ops.size() * sizeof(DevOp), hipMemcpyHostToDevice, c.stream),
+                "hipMemcpyAsync(op tables)");
+    PassArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.r = p.shape.r; a.c = p.shape.c; a.B = p.shape.B; a.R = p.shape.R;
+    a.n_tiles = int64_t(1) << (p.qubits - p.shape.r);
+    const uint64_t all = (uint64_t(1) << p.qubits) - 1;
+    HGM_HIP(hipEventRecord(c.e0, c.stream), "hipEventRecord");
+    for (const Pass &ps : p.passes) {
+        if (ps.op_count == 0) continue;
+        a.nonlocal_mask = all & ~ps.local_mask;
+        std::memcpy(a.local_pos, ps.local_pos, sizeof(a.local_pos));
+        HGM_HIP(launch_pass_lds(p.dtype, s->d, c.ops + ps.op_begin, (int)ps.op_count, a, c.stream), "pass launch");
+    }
+    HGM_HIP(hipEventRecord(c.e1, c.stream), "hipEventRecord");
+    HGM_HIP(hipEventSynchronize(c.e1), "pass execution");
+    if (st) {
+        int64_t launched = 0;
+        for (const Pass &ps : p.passes) launched += ps.op_count > 0;
+        st->gates += p.gates;
+        st->passes += launched;
+        st->amp_updates += p.gates * (int64_t(1) << p.qubits);
+        st->bytes_alg += (double)launched * 2.0 * (double)(uint64_t(1) << p.qubits) * (double)amp_bytes(p.dtype);
+        st->plan_ms += p.plan_ms;
+        st->kernel_ms += elapsed(c.e0, c.e1);
+    }
+    return HGM_OK;
+}
+
+int blocks_for(uint64_t n) {
+    uint64_t b = (n + 255) / 256;
+    return (int)(b < (uint64_t)kRedBlocks ? (b ? b : 1) : kRedBlocks);
+}
+
+int prob_halves_locked(const hgm_state *s, int bit, double *zero, double *one) {
+    Ctx &c = ctx();
+    const uint64_t n = uint64_t(1) << s->qubits;
+    const int blocks = blocks_for(n);
+    HGM_HIP(launch_prob_halves(s->dtype, s->d, n, bit, c.partial, blocks, c.stream), "prob kernel");
+    HGM_HIP(hipMemcpyAsync(c.h_partial, c.partial, 2 * blocks * sizeof(double), hipMemcpyDeviceToHost, c.stream),
+            "hipMemcpyAsync(partials)");
+    HGM_HIP(hipStreamSynchronize(c.stream), "prob kernel");
+    double z = 0, o = 0;
+    for (int b = 0; b < blocks; ++b) { z += c.h_partial[2 * b]; o += c.h_partial[2 * b + 1]; }
+    *zero = z;
+    *one = o;
+    return HGM_OK;
+}
+
+}  // namespace
+
+void set_error(const std::string &msg) { t_err = msg; }
+const std::string &last_error() { return t_err; }
+
+int gpu_init(int device) {
+    std::lock_guard<std::mutex> g(ctx().mu);
+    return init_locked(device);
+}
+
+int alloc_pinned(void **p, size_t bytes) {
+    if (hipHostMalloc(p, bytes, hipHostMallocDefault) != hipSuccess) {
+        *p = nullptr;
+        return HGM_E_NOMEM;
+    }
+    std::memset(*p, 0, bytes);
+    return HGM_OK;
+}
+
+void free_pinned(void *p) {
+    if (p) hipHostFree(p);
+}
+
+int plan_and_run(hgm_state *s, const hgm_op *ops, int64_t n_ops, const hgm_tuning *tuning, int iterations,
+                 hgm_stats *st) {
+    hgm_tuning t;
+    if (tuning) t = *tuning; else hgm_default_tuning(&t);
+    Plan p;
+    int rc = build_plan(s->qubits, s->dtype, ops, n_ops, iterations, t, &p, &t_err);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(ctx().mu);
+    rc = init_locked(-1);
+    if (rc) return rc;
+    return run_plan_locked(s, p, st);
+}
+
+}  // namespace hgm
+
+using namespace hgm;
+
+extern "C" {
+
+void hgm_default_tuning(hgm_tuning *t) {
+    std::memset(t, 0, sizeof(*t));
+    t->block_size = 64;          // include/cli/cli_common.h:14-16
+    t->repeat_count = 2;
+    t->gpu_region_bits = 8;
+    t->gpu_coalesced_bits = 4;
+    t->ops_block = 300;          // makefile:7
+    t->planner = HGM_PLAN_GREEDY;
+    t->kernel = HGM_KERNEL_AUTO;
+    t->lookahead = 0;
+}
+
+const char *hgm_last_error(void) { return t_err.c_str(); }
+
+const char *hgm_version(void) { return "hgm-mi355x 0.1 (gfx950)"; }
+
+int hgm_init(int device) { return gpu_init(device); }
+
+int hgm_device_count(int *count) {
+    if (!count) return HGM_E_ARG;
+    if (hipGetDeviceCount(count) != hipSuccess) *count = 0;
+    return HGM_OK;
+}
+
+int hgm_release_cache(void) {
+    Ctx &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (c.cache) { hipStreamSynchronize(c.stream); hipFree(c.cache); c.cache = nullptr; c.cache_cap = 0; }
+    if (c.ops) { hipStreamSynchronize(c.stream); hipFree(c.ops); c.ops = nullptr; c.ops_cap = 0; }
+    return HGM_OK;
+}
+
+int hgm_validate_tuning(int qubits, int dtype, const hgm_tuning *t) {
+    if (!t) { t_err = "null tuning"; return HGM_E_ARG; }
+    TileShape s;
+    return make_shape(qubits, dtype, *t, &s, &t_err);
+}
+
+int hgm_state_create(int qubits, int dtype, hgm_state **out) {
+    if (!out || qubits < 1 || qubits > 40 || (dtype != HGM_C128 && dtype != HGM_C64)) {
+        t_err = "hgm_state_create: bad arguments";
+        return HGM_E_ARG;
+    }
+    Ctx &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(-1);
+    if (rc) return rc;
+    hgm_state *s = new hgm_state();
+    s->qubits = qubits;
+    s->dtype = dtype;
+    s->bytes = (size_t(1) << qubits) * amp_bytes(dtype);
+    if (hipMalloc(&s->d, s->bytes) != hipSuccess) {
+        delete s;
+        hipGetLastError();
+        t_err = "hipMalloc of a " + std::to_string(qubits) + "-qubit state failed";
+        return HGM_E_NOMEM;
+    }
+    hipError_t e = launch_set_basis(dtype, s->d, uint64_t(1) << qubits, ~uint64_t(0), c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) { hipFree(s->d); delete s; return hip_fail(e, "state clear"); }
+    *out = s;
+    return HGM_OK;
+}
+
+int hgm_state_wrap(void *device_ptr, int qubits, int dtype, hgm_state **out) {
+    if (!out || !device_ptr || qubits < 1 || qubits > 40 || (dtype != HGM_C128 && dtype != HGM_C64)) {
+        t_err = "hgm_state_wrap: bad arguments";
+        return HGM_E_ARG;
+    }
+    int rc = gpu_init(-1);
+    if (rc) return rc;
+    hgm_state *s = new hgm_state();
+    s->d = device_ptr;
+    s->qubits = qubits;
+    s->dtype = dtype;
+    s->owned = false;
+    s->bytes = (size_t(1) << qubits) * amp_bytes(dtype);
+    *out = s;
+    return HGM_OK;
+}
+
+void hgm_state_destroy(hgm_state *s) {
+    if (!s) return;
+    if (s->owned && s->d) {
+        std::lock_guard<std::mutex> g(ctx().mu);
+        hipStreamSynchronize(ctx().stream);
+        hipFree(s->d);
+    }
+    delete s;
+}
+
+int hgm_state_qubits(const hgm_state *s) { return s ? s->qubits : -1; }
+int hgm_state_dtype(const hgm_state *s) { return s ? s->dtype : -1; }
+void *hgm_state_device_ptr(hgm_state *s) { return s ? s->d : nullptr; }
+
+int hgm_state_set_basis(hgm_state *s, uint64_t index) {
+    if (!s || index >= (uint64_t(1) << s->qubits)) { t_err = "basis index outside the state"; return HGM_E_ARG; }
+    Ctx &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    HGM_HIP(launch_set_basis(s->dtype, s->d, uint64_t(1) << s->qubits, index, c.stream), "set_basis");
+    HGM_HIP(hipStreamSynchronize(c.stream), "set_basis");
+    return HGM_OK;
+}
+
+int hgm_state_upload(hgm_state *s, const void *host) {
+    if (!s || !host) { t_err = "null argument"; return HGM_E_ARG; }
+    Ctx &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    HGM_HIP(hipMemcpyAsync(s->d, host, s->bytes, hipMemcpyHostToDevice, c.stream), "upload");
+    HGM_HIP(hipStreamSynchronize(c.stream), "upload");
+    return HGM_OK;
+}
+
+int hgm_state_download(const hgm_state *s, void *host) {
+    if (!s || !host) { t_err = "null argument"; return HGM_E_ARG; }
+    Ctx &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    HGM_HIP(hipMemcpyAsync(host, s->d, s->bytes, hipMemcpyDeviceToHost, c.stream), "download");
+    HGM_HIP(hipStreamSynchronize(c.stream), "download");
+    return HGM_OK;
+}
+
+int hgm_state_apply(hgm_state *s, const hgm_op *ops, int64_t n_ops, const hgm_tuning *tuning, int iterations,
+                    hgm_stats *stats) {
+    if (!s) { t_err = "null state"; return HGM_E_ARG; }
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = plan_and_run(s, ops, n_ops, tuning, iterations, stats);
+    if (stats) stats->total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+int hgm_state_norm2(const hgm_state *s, double *out) {
+    if (!s || !out) { t_err = "null argument"; return HGM_E_ARG; }
+    std::lock_guard<std::mutex> g(ctx().mu);
+    double z, o;
+    int rc = prob_halves_locked(s, -1, &z, &o);
+    if (!rc) *out = z + o;
+    return rc;
+}
+
+int hgm_state_prob_one(const hgm_state *s, int qubit, double *out) {
+    if (!s || !out || qubit < 0 || qubit >= s->qubits) { t_err = "bad qubit"; return HGM_E_ARG; }
+    std::lock_guard<std::mutex> g(ctx().mu);
+    double z, o;
+    int rc = prob_halves_locked(s, s->qubits - 1 - qubit, &z, &o);
+    if (!rc) *out = o;
+    return rc;
+}
+
+int hgm_state_collapse_reset(hgm_state *s, int qubit, int outcome, double p) {
+    if (!s || qubit < 0 || qubit >= s->qubits || (outcome != 0 && outcome != 1) || !(p > 0)) {
+        t_err = "bad collapse arguments";
+        return HGM_E_ARG;
+    }
+    Ctx &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    HGM_HIP(launch_collapse(s->dtype, s->d, uint64_t(1) << s->qubits, s->qubits - 1 - qubit, outcome, std::sqrt(p),
+                            c.stream), "collapse");
+    HGM_HIP(hipStreamSynchronize(c.stream), "collapse");
+    return HGM_OK;
+}
+
+int hgm_state_argmax(const hgm_state *s, uint64_t *index, double *prob) {
+    if (!s || !index) { t_err = "null argument"; return HGM_E_ARG; }
+    Ctx &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    const uint64_t n = uint64_t(1) << s->qubits;
+    const int blocks = blocks_for(n);
+    HGM_HIP(launch_argmax(s->dtype, s->d, n, c.partial, c.pidx, blocks, c.stream), "argmax");
+    HGM_HIP(hipMemcpyAsync(c.h_partial, c.partial, blocks * sizeof(double), hipMemcpyDeviceToHost, c.stream), "argmax");
+    HGM_HIP(hipMemcpyAsync(c.h_pidx, c.pidx, blocks * sizeof(uint64_t), hipMemcpyDeviceToHost, c.stream), "argmax");
+    HGM_HIP(hipStreamSynchronize(c.stream), "argmax");
+    double best = -1;
+    uint64_t bi = 0;
+    for (int b = 0; b < blocks; ++b)
+        if (c.h_partial[b] > best || (c.h_partial[b] == best && c.h_pidx[b] < bi)) { best = c.h_partial[b]; bi = c.h_pidx[b]; }
+    *index = bi;
+    if (prob) *prob = best;
+    return HGM_OK;
+}
+
+int hgm_plan_build(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, const hgm_tuning *tuning,
+                   hgm_plan **out) {
+    if (!out) { t_err = "null output"; return HGM_E_ARG; }
+    hgm_tuning t;
+    if (tuning) t = *tuning; else hgm_default_tuning(&t);
+    hgm_plan *p = new hgm_plan();
+    int rc = build_plan(qubits, dtype, ops, n_ops, iterations, t, &p->plan, &t_err);
+    if (rc) { delete p; return rc; }
+    *out = p;
+    return HGM_OK;
+}
+
+void hgm_plan_destroy(hgm_plan *p) { delete p; }
+
+int64_t hgm_plan_num_passes(const hgm_plan *p) { return p ? (int64_t)p->plan.passes.size() : -1; }
+
+int hgm_plan_pass(const hgm_plan *p, int64_t i, uint64_t *local_mask, int64_t *n_gates, int64_t *gate_index) {
+    if (!p || i < 0 || i >= (int64_t)p->plan.passes.size()) { t_err = "pass index out of range"; return HGM_E_ARG; }
+    const Pass &ps = p->plan.passes[i];
+    if (local_mask) *local_mask = ps.local_mask;
+    if (n_gates) *n_gates = ps.op_count;
+    if (gate_index)
+        for (int64_t k = 0; k < ps.op_count; ++k) gate_index[k] = p->plan.src_index[ps.op_begin + k];
+    return HGM_OK;
+}
+
+int hgm_plan_shape(const hgm_plan *p, int32_t *region_bits, int32_t *block_size, int32_t *repeat_count,
+                   int32_t *coalesced_bits) {
+    if (!p) { t_err = "null plan"; return HGM_E_ARG; }
+    if (region_bits) *region_bits = p->plan.shape.r;
+    if (block_size) *block_size = p->plan.shape.B;
+    if (repeat_count) *repeat_count = p->plan.shape.R;
+    if (coalesced_bits) *coalesced_bits = p->plan.shape.c;
+    return HGM_OK;
+}
+
+int hgm_state_run_plan(hgm_state *s, const hgm_plan *p, hgm_stats *stats) {
+    if (!s || !p) { t_err = "null argument"; return HGM_E_ARG; }
+    auto t0 = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> g(ctx().mu);
+    int rc = init_locked(-1);
+    if (!rc) rc = run_plan_locked(s, p->plan, stats);
+    if (stats) stats->total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+int hgm_execute(void *host_state, int qubits, int dtype, const hgm_op *ops, int64_t n_ops, const hgm_tuning *tuning,
+                int iterations, hgm_stats *stats) {
+    if (!host_state) { t_err = "null host state"; return HGM_E_ARG; }
+    auto t0 = std::chrono::steady_clock::now();
+    hgm_tuning t;
+    if (tuning) t = *tuning; else hgm_default_tuning(&t);
+    Plan p;
+    int rc = build_plan(qubits, dtype, ops, n_ops, iterations, t, &p, &t_err);
+    if (rc) return rc;
+    Ctx &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    rc = init_locked(-1);
+    if (rc) return rc;
+    const size_t bytes = (size_t(1) << qubits) * amp_bytes(dtype);
+    if (bytes > c.cache_cap) {
+        if (c.cache) { hipStreamSynchronize(c.stream); hipFree(c.cache); c.cache = nullptr; c.cache_cap = 0; }
+        if (hipMalloc(&c.cache, bytes) != hipSuccess) {
+            hipGetLastError();
+            t_err = "hipMalloc of a " + std::to_string(qubits) + "-qubit state failed";
+            return HGM_E_NOMEM;
+        }
+        c.cache_cap = bytes;
+    }
+    hgm_state s;
+    s.d = c.cache;
+    s.qubits = qubits;
+    s.dtype = dtype;
+    s.owned = false;
+    s.bytes = bytes;
+    HGM_HIP(hipEventRecord(c.e2, c.stream), "hipEventRecord");
+    HGM_HIP(hipMemcpyAsync(s.d, host_state, bytes, hipMemcpyHostToDevice, c.stream), "H2D state");
+    HGM_HIP(hipEventRecord(c.e3, c.stream), "hipEventRecord");
+    rc = run_plan_locked(&s, p, stats);
+    if (rc) return rc;
+    HGM_HIP(hipEventRecord(c.e4, c.stream), "hipEventRecord");
+    HGM_HIP(hipMemcpyAsync(host_state, s.d, bytes, hipMemcpyDeviceToHost, c.stream), "D2H state");
+    HGM_HIP(hipEventRecord(c.e5, c.stream), "hipEventRecord");
+    HGM_HIP(hipEventSynchronize(c.e5), "D2H state");
+    if (stats) {
+        stats->h2d_ms += elapsed(c.e2, c.e3);
+        stats->d2h_ms += elapsed(c.e4, c.e5);
+        stats->total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return HGM_OK;
+}
+
+}  // extern "C"

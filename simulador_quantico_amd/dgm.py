@@ -1,0 +1,297 @@
+"""Python mirror of the reference's DGM executor (include/core/dgm.h:65-163).
+
+Every method forwards to the C++ executor inside libhgm.so through the C ABI
+(include/hgm_gpu.h); only exec_type == T_GPU executes. Method names, argument
+meanings and defaults follow the reference so that a reference test reads
+the same here::
+
+    dgm = DGM()
+    dgm.qubits = 3
+    dgm.exec_type = T_GPU
+    dgm.allocateMemory()
+    dgm.setMemoryValue(5)
+    dgm.executeFunction(Toffoli(3, 0, 1, 2, 3))
+    dgm.state[7]
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from ._native import HGMError, Op, Stats, check, lib
+
+T_CPU, T_PAR_CPU, T_GPU, T_HYBRID = 0, 1, 2, 3      # include/core/dgm.h:42-47
+DENSE, DIAG_PRI, DIAG_SEC = 0, 1, 2                  # include/core/common.h:48-52
+
+_INT_FIELDS = ("qubits", "exec_type", "gpu_count", "block_size", "repeat_count", "gpu_region_bits",
+               "gpu_coalesced_bits", "ops_block", "planner", "kernel", "thread_count", "cpu_region_bits",
+               "cpu_coalesced_bits", "resident")
+
+
+def _m8(m) -> "C.Array":
+    m = np.asarray(m, dtype=np.complex128).reshape(4)
+    return (C.c_double * 8)(*np.column_stack([m.real, m.imag]).ravel())
+
+
+class _Gates:
+    """The DGM's gate catalog (include/core/gates.h:40-54)."""
+
+    def __init__(self, owner: "DGM"):
+        self._o = owner
+
+    def addGate(self, name: str, *m) -> bool:
+        mat = m[0] if len(m) == 1 else m
+        added = C.c_int(0)
+        check(lib.hgm_dgm_add_gate(self._o._h, name.encode(), _m8(mat), C.byref(added)))
+        return bool(added.value)
+
+    def getMatrix(self, name: str):
+        buf = (C.c_double * 8)()
+        found = C.c_int(0)
+        check(lib.hgm_dgm_get_gate(self._o._h, name.encode(), buf, C.byref(found)))
+        if not found.value:
+            return None
+        a = np.frombuffer(buf, dtype=np.float64).copy()
+        return a[0::2] + 1j * a[1::2]
+
+
+class DGM:
+    """Executor handle. dtype: np.complex128 (default) or np.complex64."""
+
+    def __init__(self, dtype=np.complex128):
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.complex128, np.complex64):
+            raise ValueError("dtype must be complex128 or complex64")
+        h = C.c_void_p()
+        check(lib.hgm_dgm_create(N.HGM_C128 if self.dtype == np.complex128 else N.HGM_C64, C.byref(h)))
+        self._h = h
+        self.gates = _Gates(self)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.hgm_dgm_destroy(h)
+            self._h = None
+
+    def __getattr__(self, name):
+        if name in _INT_FIELDS:
+            v = C.c_int64()
+            check(lib.hgm_dgm_get(self.__dict__["_h"], name.encode(), C.byref(v)))
+            return v.value
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        if name in _INT_FIELDS:
+            check(lib.hgm_dgm_set(self._h, name.encode(), int(value)))
+        else:
+            object.__setattr__(self, name, value)
+
+    # ---- state -------------------------------------------------------
+    def allocateMemory(self) -> None:
+        check(lib.hgm_dgm_allocate_memory(self._h))
+
+    def freeMemory(self) -> None:
+        check(lib.hgm_dgm_free_memory(self._h))
+
+    def setMemoryValue(self, pos: int) -> None:
+        check(lib.hgm_dgm_set_memory_value(self._h, int(pos)))
+
+    @property
+    def state(self) -> np.ndarray:
+        """Zero-copy view of the host amplitudes (valid until freeMemory)."""
+        p = lib.hgm_dgm_state(self._h)
+        if not p:
+            raise HGMError(6, "no state allocated")
+        n = 1 << self.qubits
+        buf = (C.c_byte * (n * self.dtype.itemsize)).from_address(p)
+        return np.frombuffer(buf, dtype=self.dtype)
+
+    def syncHost(self) -> None:
+        check(lib.hgm_dgm_sync_host(self._h))
+
+    # ---- circuits ----------------------------------------------------
+    def setFunction(self, function, iterations: int = 1, reset: bool = True) -> None:
+        if isinstance(function, str):
+            check(lib.hgm_dgm_set_function(self._h, function.encode(), iterations, int(reset)))
+        else:
+            steps = [s.encode() for s in function]
+            arr = (C.c_char_p * max(1, len(steps)))(*steps)
+            check(lib.hgm_dgm_set_function_steps(self._h, arr, len(steps), iterations, int(reset)))
+
+    def ops(self) -> "C.Array":
+        n = C.c_int64()
+        check(lib.hgm_dgm_num_pts(self._h, C.byref(n)))
+        arr = (Op * max(1, n.value))()
+        check(lib.hgm_dgm_get_ops(self._h, arr, n.value))
+        return arr if n.value else (Op * 0)()
+
+    def pts(self):
+        """The compiled PT list as (target_bits, control_masks, control_values, matrices[n,4])."""
+        return ops_to_arrays(self.ops())
+
+    def execute(self, iterations: int = 1) -> np.ndarray:
+        check(lib.hgm_dgm_execute(self._h, iterations))
+        return self.state
+
+    def executeFunction(self, function, iterations: int = 1) -> None:
+        if isinstance(function, str):
+            if function == "":
+                return
+            check(lib.hgm_dgm_execute_function(self._h, function.encode(), iterations))
+        else:
+            self.setFunction(function)
+            self.execute(iterations)
+
+    # ---- readout -----------------------------------------------------
+    def measure(self, qubit):
+        if isinstance(qubit, (list, tuple)):
+            qs = (C.c_int * max(1, len(qubit)))(*qubit)
+            n = C.c_int64(0)
+            check(lib.hgm_dgm_measure_many(self._h, qs, len(qubit), None, None, C.byref(n)))
+            vals = (C.c_int64 * max(1, n.value))()
+            probs = (C.c_double * max(1, n.value))()
+            check(lib.hgm_dgm_measure_many(self._h, qs, len(qubit), vals, probs, C.byref(n)))
+            return {int(vals[i]): float(probs[i]) for i in range(n.value)}
+        out = C.c_int()
+        check(lib.hgm_dgm_measure(self._h, int(qubit), C.byref(out)))
+        return out.value
+
+    def CountOps(self, iterations: int = 1) -> dict:
+        c = (C.c_int64 * 7)()
+        check(lib.hgm_dgm_count_ops(self._h, iterations, c))
+        keys = ("total_op", "dense", "main_diag", "sec_diag", "c_dense", "c_main_diag", "c_sec_diag")
+        return dict(zip(keys, list(c)))
+
+    @property
+    def stats(self) -> dict:
+        s = Stats()
+        check(lib.hgm_dgm_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+
+def seed_rng(seed: int) -> None:
+    lib.hgm_seed_rng(int(seed))
+
+
+def ops_to_arrays(ops):
+    n = len(ops)
+    t = np.array([ops[i].target_bit for i in range(n)], dtype=np.int32)
+    cm = np.array([ops[i].control_mask for i in range(n)], dtype=np.uint64)
+    cv = np.array([ops[i].control_value for i in range(n)], dtype=np.uint64)
+    m = np.array([list(ops[i].m) for i in range(n)], dtype=np.float64).reshape(n, 8)
+    return t, cm, cv, m[:, 0::2] + 1j * m[:, 1::2]
+
+
+def make_ops(targets, masks, values, mats) -> "C.Array":
+    """Build an hgm_op array from parallel sequences (matrices as 4 complex each)."""
+    n = len(targets)
+    arr = (Op * max(1, n))()
+    for i in range(n):
+        arr[i].target_bit = int(targets[i])
+        arr[i].control_mask = int(masks[i])
+        arr[i].control_value = int(values[i])
+        m = np.asarray(mats[i], dtype=np.complex128).reshape(4)
+        for k in range(4):
+            arr[i].m[2 * k] = m[k].real
+            arr[i].m[2 * k + 1] = m[k].imag
+    return arr if n else (Op * 0)()
+
+
+class State:
+    """A device-resident state (hgm_state): apply gate lists without host copies."""
+
+    def __init__(self, qubits: int, dtype=np.complex128):
+        self.dtype = np.dtype(dtype)
+        self.qubits = qubits
+        h = C.c_void_p()
+        check(lib.hgm_state_create(qubits, N.HGM_C128 if self.dtype == np.complex128 else N.HGM_C64, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.hgm_state_destroy(h)
+            self._h = None
+
+    def set_basis(self, index: int) -> None:
+        check(lib.hgm_state_set_basis(self._h, int(index)))
+
+    def upload(self, amps) -> None:
+        a = np.ascontiguousarray(amps, dtype=self.dtype)
+        assert a.size == 1 << self.qubits
+        check(lib.hgm_state_upload(self._h, a.ctypes.data))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(1 << self.qubits, dtype=self.dtype)
+        check(lib.hgm_state_download(self._h, out.ctypes.data))
+        return out
+
+    def apply(self, ops, tuning=None, iterations: int = 1) -> dict:
+        st = Stats()
+        t = tuning if tuning is not None else N.default_tuning()
+        check(lib.hgm_state_apply(self._h, ops, len(ops), C.byref(t), iterations, C.byref(st)))
+        return st.as_dict()
+
+    def norm2(self) -> float:
+        v = C.c_double()
+        check(lib.hgm_state_norm2(self._h, C.byref(v)))
+        return v.value
+
+    def prob_one(self, qubit: int) -> float:
+        v = C.c_double()
+        check(lib.hgm_state_prob_one(self._h, qubit, C.byref(v)))
+        return v.value
+
+    def collapse_reset(self, qubit: int, outcome: int, p: float) -> None:
+        check(lib.hgm_state_collapse_reset(self._h, qubit, outcome, p))
+
+    def argmax(self):
+        i = C.c_uint64()
+        p = C.c_double()
+        check(lib.hgm_state_argmax(self._h, C.byref(i), C.byref(p)))
+        return i.value, p.value
+
+
+class Plan:
+    """A pass plan (hgm_plan) built on the host; inspectable without a GPU."""
+
+    def __init__(self, qubits: int, ops, tuning=None, iterations: int = 1, dtype=np.complex128):
+        h = C.c_void_p()
+        t = tuning if tuning is not None else N.default_tuning()
+        dt = N.HGM_C128 if np.dtype(dtype) == np.complex128 else N.HGM_C64
+        check(lib.hgm_plan_build(qubits, dt, ops, len(ops), iterations, C.byref(t), C.byref(h)))
+        self._h = h
+        self.qubits = qubits
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.hgm_plan_destroy(h)
+            self._h = None
+
+    def __len__(self) -> int:
+        return int(lib.hgm_plan_num_passes(self._h))
+
+    def shape(self):
+        r, b, rc, c = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib.hgm_plan_shape(self._h, C.byref(r), C.byref(b), C.byref(rc), C.byref(c)))
+        return r.value, b.value, rc.value, c.value
+
+    def passes(self):
+        """List of (local_mask, [gate indices in execution order]) per pass."""
+        out = []
+        for i in range(len(self)):
+            mask = C.c_uint64()
+            n = C.c_int64()
+            check(lib.hgm_plan_pass(self._h, i, C.byref(mask), C.byref(n), None))
+            idx = np.zeros(max(1, n.value), dtype=np.int64)
+            check(lib.hgm_plan_pass(self._h, i, C.byref(mask), C.byref(n), idx.ctypes.data))
+            out.append((mask.value, idx[: n.value].tolist()))
+        return out
+
+    def run(self, state: State) -> dict:
+        st = Stats()
+        check(lib.hgm_state_run_plan(state._h, self._h, C.byref(st)))
+        return st.as_dict()

@@ -1,0 +1,288 @@
+"""Parity of the HIP t_GPU path (through the C ABI) with the oracle and with
+the reference's own outputs. Tolerances: FP64 state vs FP64 oracle 1e-12
+max-abs; FP32 state vs FP64 oracle 2e-5 max-abs (single-precision rounding of
+~10^2 gates); measured/argmax indices exact."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import simulador_quantico_amd as hq
+from simulador_quantico_amd import _native as N
+from oracle import pyoracle as po
+from tests.test_oracle import _random_gate_list, grover_closed_form, oracle_run_gates, truth_table_cases
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 2e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dgm_for(n, dtype=np.complex128, **tun):
+    d = hq.DGM(dtype)
+    d.qubits = n
+    d.exec_type = hq.T_GPU
+    for k, v in tun.items():
+        setattr(d, k, v)
+    d.allocateMemory()
+    return d
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_truth_tables(dtype):
+    say("truth tables", dtype)
+    for n, circ, start, exp in truth_table_cases():
+        d = dgm_for(n, dtype)
+        d.setMemoryValue(start)
+        d.executeFunction(circ)
+        want = np.zeros(1 << n)
+        want[exp] = 1
+        assert np.abs(d.state - want).max() < 1e-6, (circ, start)
+
+
+def test_golden_circuits(golden):
+    """FP32 path vs the reference's FP32 states; FP64 path vs the FP64 oracle."""
+    for c in golden["circuits"]:
+        say("golden", c["name"])
+        n = c["qubits"]
+        ref32 = golden["states"][f"{c['name']}__tcpu"]
+        for dtype in (np.complex64, np.complex128):
+            d = dgm_for(n, dtype)
+            for name, m in c["gates"].items():
+                d.gates.addGate(name, [complex(a, b) for a, b in m])
+            d.setMemoryValue(c["basis"])
+            d.setFunction(c["steps"], c["iterations"])
+            got = d.execute(1).copy()
+            if dtype == np.complex64:
+                assert np.abs(got - ref32).max() < TOL32, c["name"]
+            else:
+                o = po.Oracle(n, 64)
+                for name, m in c["gates"].items():
+                    o.add_gate(name, [complex(a, b) for a, b in m])
+                o.set_function(c["steps"], c["iterations"])
+                o.set_basis(c["basis"])
+                o.execute(po.T_CPU)
+                assert np.abs(got - o.state()).max() < TOL64, c["name"]
+
+
+TUNINGS = [  # block_size, repeat_count, gpu_region_bits, gpu_coalesced_bits
+    (64, 2, 8, 4), (128, 2, 9, 4), (256, 8, 12, 4), (256, 16, 13, 2), (64, 1, 7, 0), (32, 4, 8, 6),
+    (1024, 4, 13, 5), (16, 1, 5, 1),
+]
+
+
+@pytest.mark.parametrize("n", [5, 9, 13, 16])
+def test_random_gates_tuning_sweep(n):
+    rng = np.random.default_rng(1000 + n)
+    t, cm, cv, mats = _random_gate_list(rng, n, 120)
+    ops = hq.make_ops(t, cm, cv, mats)
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+    want = oracle_run_gates(n, t, cm, cv, mats, state=psi0)
+    for B, R, r, c in TUNINGS:
+        for planner in (N.HGM_PLAN_GREEDY, N.HGM_PLAN_WINDOW):
+            for ops_block in (300, 5):
+                say("sweep", n, (B, R, r, c), planner, ops_block)
+                tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c,
+                                       planner=planner, ops_block=ops_block)
+                for dtype, tol in ((np.complex128, TOL64), (np.complex64, TOL32)):
+                    s = hq.State(n, dtype)
+                    s.upload(psi0)
+                    s.apply(ops, tun)
+                    assert np.abs(s.download() - want).max() < tol, (B, R, r, c, planner, ops_block, dtype)
+
+
+def test_iterations_replay_the_list():
+    """t_GPU honours `iterations` (src/core/kernel.cu:192)."""
+    n = 10
+    rng = np.random.default_rng(3)
+    t, cm, cv, mats = _random_gate_list(rng, n, 40)
+    ops = hq.make_ops(t, cm, cv, mats)
+    want = oracle_run_gates(n, list(t) * 3, list(cm) * 3, list(cv) * 3, list(mats) * 3, basis=7)
+    s = hq.State(n)
+    s.set_basis(7)
+    st = s.apply(ops, N.default_tuning(block_size=64, repeat_count=4, gpu_region_bits=9), iterations=3)
+    assert st["gates"] == 120
+    assert np.abs(s.download() - want).max() < TOL64
+
+
+def test_hadamard_20q_config():
+    """Hadamard benchmark, 20 qubits, FP64: every amplitude 2^-10 (1 layer and general.out's 3 layers)."""
+    for layers in (1, 3):
+        say("hadamard20", layers)
+        d = dgm_for(20, block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4)
+        d.setMemoryValue(0)
+        d.setFunction(hq.Hadamard(20, 0, 20), layers)
+        st = d.execute(1)
+        assert np.abs(st - 2.0 ** -10).max() < TOL64
+
+
+def test_grover_22q_config():
+    """Grover, 22 qubits, FP64, 1608 iterations: closed form within 1e-10, argmax exact."""
+    n = 22
+    marked = int(np.random.default_rng(22).integers(0, 1 << n))
+    k = hq.GroverIterations(n)
+    assert k == 1608
+    d = hq.DGM()
+    d.qubits = n
+    d.setFunction(hq.GroverIteration(n, marked))
+    ops = d.ops()
+    d.setFunction(hq.Hadamard(n, 0, n))
+    hops = d.ops()
+    tun = N.default_tuning(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=4)
+    s = hq.State(n)
+    s.set_basis(0)
+    s.apply(hops, tun)
+    st = s.apply(ops, tun, iterations=k)
+    say("grover22 passes", st["passes"], "kernel_ms", st["kernel_ms"])
+    idx, p = s.argmax()
+    assert idx == marked
+    psi = s.download()
+    assert np.abs(psi - grover_closed_form(n, marked, k)).max() < 1e-10
+    assert abs(p - np.sin((2 * k + 1) * np.arcsin(2 ** (-n / 2))) ** 2) < 1e-10
+
+
+def test_draper_adder_29q_config():
+    """2x14-bit Draper adder + carry (29 qubits): sum read bit-exact, FP64 and FP32."""
+    w = 14
+    rng = np.random.default_rng(29)
+    a, b = (int(v) for v in rng.integers(0, 1 << w, 2))
+    d = hq.DGM()
+    d.setFunction(hq.DraperAdder(w, d))
+    ops = d.ops()
+    tun = N.default_tuning(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=4)
+    for dtype, floor in ((np.complex128, 1 - 1e-10), (np.complex64, 1 - 1e-5)):
+        s = hq.State(2 * w + 1, dtype)
+        s.set_basis((a << (w + 1)) | b)
+        st = s.apply(ops, tun)
+        idx, p = s.argmax()
+        say("draper29", dtype, "passes", st["passes"], "p", p)
+        assert idx == (a << (w + 1)) | (a + b)
+        assert (idx & ((1 << (w + 1)) - 1)) == a + b
+        assert p > floor
+        assert abs(s.norm2() - 1) < (1e-12 if dtype == np.complex128 else 1e-5)
+
+
+def inverse_ops(ops):
+    """U^dagger of a gate list: reversed order, conjugate-transposed matrices."""
+    t, cm, cv, mats = hq.ops_to_arrays(ops)
+    inv = [m.reshape(2, 2).conj().T.reshape(4) for m in mats[::-1]]
+    return hq.make_ops(t[::-1], cm[::-1], cv[::-1], inv)
+
+
+def test_random_layers_20q_vs_oracle():
+    n, depth = 20, 20
+    d = hq.DGM()
+    d.setFunction(hq.RandomLayers(n, depth, 5, d))
+    ops = d.ops()
+    t, cm, cv, mats = hq.ops_to_arrays(ops)
+    want = oracle_run_gates(n, t, cm, cv, mats, basis=0, engine=po.T_PAR_CPU)
+    for tun in (N.default_tuning(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4),
+                N.default_tuning(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=2)):
+        s = hq.State(n)
+        s.set_basis(0)
+        st = s.apply(ops, tun)
+        say("random20 passes", st["passes"])
+        assert np.abs(s.download() - want).max() < TOL64
+
+
+@pytest.mark.parametrize("n,dtype,tol", [(30, np.complex128, 1e-12), (28, np.complex64, 2e-5)])
+def test_random_layers_full_size_round_trip(n, dtype, tol):
+    """U then U^dagger returns |0>; the norm stays 1 (size-independent checks at full size)."""
+    depth = 20
+    d = hq.DGM()
+    d.setFunction(hq.RandomLayers(n, depth, 1, d))
+    ops = d.ops()
+    tun = N.default_tuning(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=2)
+    s = hq.State(n, dtype)
+    s.set_basis(0)
+    st = s.apply(ops, tun)
+    say("random", n, "passes", st["passes"], "kernel_ms", st["kernel_ms"])
+    assert abs(s.norm2() - 1) < 10 * tol
+    s.apply(inverse_ops(ops), tun)
+    idx, p = s.argmax()
+    assert idx == 0
+    assert abs(p - 1) < 10 * tol
+    del s
+
+
+def test_measure_matches_reference(golden):
+    """Grover (reference form) measured on the t_GPU path: the reference's bits at its seeds."""
+    for resident in (0, 1):
+        for case in golden["measure"]:
+            n, v = case["qubits"], case["search_value"]
+            d = dgm_for(n, resident=resident)
+            d.setMemoryValue(1 << (n - 1))
+            d.setFunction(hq.Hadamard(n, 0, n))
+            d.setFunction(hq.GroverIterationRef(n, v), case["iterations"], reset=False)
+            d.execute(1)
+            hq.seed_rng(case["seed"])
+            bits = [d.measure(q) for q in range(1, n)]
+            assert bits == case["bits"], (resident, case)
+
+
+def test_device_readout():
+    n = 14
+    rng = np.random.default_rng(9)
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    s = hq.State(n)
+    s.upload(psi)
+    assert abs(s.norm2() - 1) < 1e-13
+    for q in (0, 5, 13):
+        bit = n - 1 - q
+        p1 = np.sum(np.abs(psi[(np.arange(1 << n) >> bit) & 1 == 1]) ** 2)
+        assert abs(s.prob_one(q) - p1) < 1e-13
+    idx, p = s.argmax()
+    assert idx == int(np.argmax(np.abs(psi) ** 2))
+    # collapse-and-reset against the reference's host formula (src/core/dgm_core.cpp:115-122)
+    q, out = 3, 1
+    p1 = s.prob_one(q)
+    s.collapse_reset(q, out, p1)
+    bit = n - 1 - q
+    want = np.zeros_like(psi)
+    for i in range(1 << (n - 1)):
+        pos = ((i >> bit) << (bit + 1)) | (i & ((1 << bit) - 1))
+        want[pos] = psi[pos | (out << bit)] / np.sqrt(p1)
+    assert np.abs(s.download() - want).max() < 1e-13
+
+
+class RefPT(C.Structure):
+    """The reference's struct PT on LP64 (include/core/common.h:57-90)."""
+    _fields_ = [("qubits", C.c_int), ("matrix", C.c_void_p), ("matrix_size", C.c_int),
+                ("span_start_bit", C.c_int), ("target_bit", C.c_int), ("affected", C.c_bool),
+                ("control_value", C.c_long), ("control_mask", C.c_long), ("control_bit_positions", C.c_void_p),
+                ("control_count", C.c_long), ("control_rest", C.c_void_p), ("control_rest_count", C.c_long)]
+
+
+def test_gpu_execution_wrapper_reference_abi():
+    """The reference-compatible entry point on an FP32 host state and a PT** list."""
+    assert C.sizeof(RefPT) == 80
+    n = 11
+    rng = np.random.default_rng(77)
+    t, cm, cv, mats = _random_gate_list(rng, n, 50)
+    mats32 = [np.asarray(m, np.complex64) for m in mats]
+    keep = []
+    pts = (C.POINTER(RefPT) * (len(t) + 1))()
+    for i in range(len(t)):
+        m = np.ascontiguousarray(mats32[i])
+        keep.append(m)
+        p = RefPT(qubits=1, matrix=m.ctypes.data, matrix_size=2, span_start_bit=int(t[i]) + 1,
+                  target_bit=int(t[i]), affected=False, control_value=int(cv[i]), control_mask=int(cm[i]),
+                  control_count=bin(int(cm[i])).count("1"))
+        keep.append(p)
+        pts[i] = C.pointer(p)
+    pts[len(t)] = None
+    state = np.zeros(1 << n, np.complex64)
+    state[5] = 1
+    ret = N.lib.GpuExecutionWrapper(state.ctypes.data, C.cast(pts, C.c_void_p), n, 4, 8, 1, 64, 2, 2)
+    assert ret == state.ctypes.data
+    want = oracle_run_gates(n, list(t) * 2, list(cm) * 2, list(cv) * 2,
+                            [m.astype(np.complex128) for m in mats32] * 2, basis=5)
+    assert np.abs(state - want).max() < TOL32

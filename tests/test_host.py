@@ -1,0 +1,246 @@
+"""Host-side (no GPU) checks of the library: generators, parser, tuning
+validation, error behaviour, and the exported C ABI."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import simulador_quantico_amd as hq
+from simulador_quantico_amd import _native as N
+from oracle import pyoracle as po
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ------------------------------------------------------------- C ABI surface
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "hgm_gpu.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(hgm_[a-z0-9_]+|setDevice|GpuExecutionWrapper|ProjectState|GetState)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 50
+    lib = C.CDLL(N.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+    bound = {s[0] for s in N.SIGNATURES}
+    assert set(names) == bound, set(names) ^ bound
+
+
+def test_version_and_no_device_here():
+    assert b"gfx950" in N.lib.hgm_version()
+
+
+# ------------------------------------------------------------- generators
+def _product_gen(which, args, op, dgm):
+    return hq.circuits._gen(which, args, op, dgm)
+
+
+def test_generators_match_reference_text(golden):
+    """Every generator output equals the reference's byte for byte."""
+    d = hq.DGM()
+    n = 0
+    for s in golden["strings"]:
+        got = _product_gen(s["which"], s["args"], s["op"], d)
+        assert got == s["text"], (s["which"], s["args"])
+        n += 1
+    assert n > 100
+
+
+def test_qft_is_the_dft():
+    """QFT steps of the builder on an m-qubit register = DFT with reversed output bits."""
+    m = 5
+    d = hq.DGM()
+    steps = hq.QFT(m, 1, 0, m - 1, d)
+    o = po.Oracle(m, 64)
+    d.setFunction(steps)
+    t, cm, cv, mats = d.pts()
+    from tests.test_oracle import oracle_run_gates
+    N_ = 2 ** m
+    for x in [0, 1, 5, 17, 31]:
+        got = oracle_run_gates(m, t, cm, cv, mats, basis=x)
+        rev = lambda k: int(format(k, f"0{m}b")[::-1], 2)  # noqa: E731
+        want = np.zeros(N_, complex)
+        for k in range(N_):
+            want[rev(k)] = np.exp(2j * np.pi * x * k / N_) / np.sqrt(N_)
+        assert np.abs(got - want).max() < 1e-12
+    del o
+
+
+@pytest.mark.parametrize("width", [2, 3, 4])
+def test_draper_adder_adds(width):
+    d = hq.DGM()
+    steps = hq.DraperAdder(width, d)
+    d.setFunction(steps)
+    n = 2 * width + 1
+    t, cm, cv, mats = d.pts()
+    from tests.test_oracle import oracle_run_gates
+    rng = np.random.default_rng(width)
+    for _ in range(6):
+        a, b = (int(v) for v in rng.integers(0, 1 << width, 2))
+        basis = (a << (width + 1)) | b
+        st = oracle_run_gates(n, t, cm, cv, mats, basis=basis)
+        k = int(np.argmax(np.abs(st)))
+        assert k == (a << (width + 1)) | (a + b)
+        assert abs(abs(st[k]) - 1) < 1e-12
+
+
+def test_u3_and_random_layers():
+    u = np.array(hq.dgm.DGM().gates.getMatrix("H"))
+    assert np.allclose(u, [2 ** -0.5, 2 ** -0.5, 2 ** -0.5, -2 ** -0.5])
+    d = hq.DGM()
+    steps = hq.RandomLayers(30, 20, 7, d)
+    d.setFunction(steps)
+    t, cm, cv, mats = d.pts()
+    assert len(t) == 600 + 290              # 30 U3 per layer + 15/14 CZ
+    for m in mats[:600:37]:
+        U = m.reshape(2, 2)
+        assert np.allclose(U.conj().T @ U, np.eye(2), atol=1e-14)
+    cz = mats[np.array([bin(int(x)).count("1") for x in cm]) == 1]
+    assert len(cz) == 290 and np.allclose(cz, [1, 0, 0, -1])
+    # same seed, same circuit
+    d2 = hq.DGM()
+    assert hq.RandomLayers(30, 20, 7, d2) == steps
+    assert np.array_equal(d2.gates.getMatrix("U3_19_29"), d.gates.getMatrix("U3_19_29"))
+
+
+def test_grover_iteration_count():
+    assert hq.GroverIterations(22) == 1608
+
+
+# ------------------------------------------------------------- parser
+def test_parser_matches_reference(golden):
+    for case in golden["parse"]:
+        d = hq.DGM()
+        d.setFunction(case["text"].split(";"), case.get("iterations", 1))
+        if "append" in case:
+            d.setFunction(case["append"], case["append_iterations"], reset=False)
+        t, cm, cv, _ = d.pts()
+        assert t.tolist() == case["target_bits"], case["text"]
+        assert cm.astype(np.int64).tolist() == case["control_masks"], case["text"]
+        assert cv.astype(np.int64).tolist() == case["control_values"], case["text"]
+
+
+def test_parser_matches_oracle_on_random_text():
+    from tests.golden.make_golden import custom_gates, random_circuit
+    rng = np.random.default_rng(11)
+    for n in (3, 6, 9, 16):
+        g = custom_gates(rng)
+        steps = random_circuit(rng, n, 8)
+        d = hq.DGM()
+        o = po.Oracle(n, 64)
+        for k, m in g.items():
+            d.gates.addGate(k, m)
+            o.add_gate(k, m)
+        d.setFunction(steps, 2)
+        o.set_function(steps, 2)
+        t, cm, cv, mats = d.pts()
+        t2, cm2, cv2, m2 = o.gates()
+        assert t.tolist() == t2.tolist()
+        assert cm.astype(np.int64).tolist() == cm2.tolist() and cv.astype(np.int64).tolist() == cv2.tolist()
+        assert np.array_equal(mats, m2[:, 0::2] + 1j * m2[:, 1::2])
+
+
+def test_unknown_gate_is_an_error():
+    d = hq.DGM()
+    with pytest.raises(hq.HGMError, match="unknown gate 'FOO'"):
+        d.setFunction("H,FOO")
+
+
+def test_addgate_first_writer_wins():
+    d = hq.DGM()
+    assert d.gates.addGate("G", [1, 0, 0, 1j])
+    assert not d.gates.addGate("G", [0, 1, 1, 0])
+    assert np.allclose(d.gates.getMatrix("G"), [1, 0, 0, 1j])
+    assert not d.gates.addGate("H", [1, 0, 0, 1])          # built-ins are not overwritten
+
+
+def test_catalog_phases_in_double():
+    d = hq.DGM()
+    for k in (1, 2, 3):
+        assert abs(d.gates.getMatrix(f"R{k}")[3] - np.exp(1j * np.pi / 2 ** (k - 1))) < 1e-16
+
+
+def test_count_ops():
+    d = hq.DGM()
+    d.setFunction(["H,X,Z", "Control1(1),Target1(Y),ID", "Control1(0),ID,Target1(R2)"], 2)
+    c = d.CountOps(3)
+    assert c["dense"] == 2 * 3 * 1 and c["sec_diag"] == 2 * 3 and c["main_diag"] == 2 * 3
+    assert c["c_sec_diag"] == 6 and c["c_main_diag"] == 6 and c["total_op"] == 30
+
+
+# ------------------------------------------------------------- tuning / errors
+def tuning_rc(q, **kw):
+    t = N.default_tuning(**kw)
+    return N.lib.hgm_validate_tuning(q, N.HGM_C128, C.byref(t))
+
+
+def test_tuning_contract():
+    assert tuning_rc(20) == 0                                           # reference defaults
+    assert tuning_rc(20, block_size=128, repeat_count=2, gpu_region_bits=9) == 0
+    # src/core/dgm_core.cpp:207: 2*128*4 != 2^8 (docs/04-gpu-cuda.md:139-142)
+    assert tuning_rc(20, block_size=128, repeat_count=4, gpu_region_bits=8) == 2
+    assert b"2*block_size*repeat_count" in N.lib.hgm_last_error()
+    assert tuning_rc(20, gpu_coalesced_bits=9) == 2                     # r < c
+    assert tuning_rc(20, gpu_coalesced_bits=8) == 2                     # r == c loops forever in the reference
+    assert tuning_rc(8, gpu_coalesced_bits=8) == 0                      # fine when the state is one tile
+    assert tuning_rc(20, block_size=2048, repeat_count=2, gpu_region_bits=13) == 2
+    assert tuning_rc(20, block_size=256, repeat_count=32, gpu_region_bits=14) == 2   # LDS (FP64)
+    t = N.default_tuning(block_size=256, repeat_count=32, gpu_region_bits=14)
+    assert N.lib.hgm_validate_tuning(20, N.HGM_C64, C.byref(t)) == 0    # FP32 fits 160 KiB
+    assert tuning_rc(20, ops_block=0) == 2
+    assert tuning_rc(3) == 0                                            # small states run as one tile
+
+
+def test_exec_types_other_than_gpu_are_rejected():
+    d = hq.DGM()
+    d.qubits = 3
+    d.allocateMemory()
+    for et, msg in [(hq.T_CPU, "only t_GPU"), (hq.T_PAR_CPU, "only t_GPU"), (hq.T_HYBRID, "only t_GPU"),
+                    (7, "Erro exec type")]:
+        d.exec_type = et
+        d.setFunction("H,H,H")
+        with pytest.raises(hq.HGMError, match=msg):
+            d.execute(1)
+    d.exec_type = hq.T_GPU
+    d.gpu_count = 2
+    with pytest.raises(hq.HGMError, match="gpu_count"):
+        d.execute(1)
+
+
+def test_circuit_width_must_match_state():
+    d = hq.DGM()
+    d.qubits = 4
+    d.allocateMemory()
+    d.setFunction("H,H,H")
+    with pytest.raises(hq.HGMError, match="3 qubits but the state was allocated for 4"):
+        d.execute(1)
+
+
+def test_ref_stubs_report_hybrid_unsupported(capfd):
+    assert N.lib.ProjectState(None, 4, 2, 0, 0, 1) is False
+    assert N.lib.GetState(None, 4, 2, 0, 0, 1) is False
+    assert "t_HYBRID" in capfd.readouterr().err
+
+
+def test_measure_host_path_matches_reference(golden):
+    """DGM.measure on a host state follows the reference's draw order (no GPU needed)."""
+    for case in golden["measure"]:
+        n, v = case["qubits"], case["search_value"]
+        o = po.Oracle(n, 64)
+        o.set_function([",".join(["H"] * n)])
+        o.set_function(hq.GroverIterationRef(n, v), case["iterations"], reset=False)
+        o.set_basis(1 << (n - 1))
+        o.execute(po.T_CPU)
+        d = hq.DGM()
+        d.qubits = n
+        d.allocateMemory()
+        d.state[:] = o.state()
+        hq.seed_rng(case["seed"])
+        bits = [d.measure(q) for q in range(1, n)]
+        assert bits == case["bits"]
