@@ -33,7 +33,9 @@ METRIC = "amp-updates/s + %HBM roofline @30q; circuits/s and p99 latency on serv
 WORKLOADS = {
     # name: (qubits, dtype, builder, tuning)
     "random30": dict(qubits=30, dtype="c128", kind="random", depth=20, seed=1,
-                     tuning=dict(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=2)),
+                     tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=2)),
+    "u3layers30": dict(qubits=30, dtype="c128", kind="u3layers", depth=20, seed=1,
+                       tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=2)),
     "random32f": dict(qubits=32, dtype="c64", kind="random", depth=20, seed=1,
                       tuning=dict(block_size=256, repeat_count=32, gpu_region_bits=14, gpu_coalesced_bits=3)),
     "hadamard20": dict(qubits=20, dtype="c128", kind="hadamard", layers=1,
@@ -55,6 +57,10 @@ def build_circuit(w):
     if w["kind"] == "random":
         d.setFunction(hq.RandomLayers(n, w["depth"], w["seed"], d))
         name = f"random-layer {n}q depth {w['depth']}"
+    elif w["kind"] == "u3layers":
+        steps = hq.RandomLayers(n, w["depth"], w["seed"], d)
+        d.setFunction([s for s in steps if s.startswith("U3_")])
+        name = f"U3 layers without CZ {n}q depth {w['depth']}"
     elif w["kind"] == "hadamard":
         d.setFunction(hq.Hadamard(n, 0, n), w["layers"])
         name = f"Hadamard layer {n}q x{w['layers']}"
@@ -126,6 +132,10 @@ def main():
     ap.add_argument("--planner", type=int, default=None)
     ap.add_argument("--ops-block", type=int, default=None)
     ap.add_argument("--tuning", default=None, help="B,R,r,c override")
+    ap.add_argument("--kernel", type=int, default=None, help="0 auto, 1 LDS, 2 register")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="time a plain device copy of the workload's state (the known-size reference for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) instead of the circuit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the multi-GPU path is a "
@@ -134,6 +144,22 @@ def main():
 
     import simulador_quantico_amd as hq
     from simulador_quantico_amd import _native as N
+
+    if args.copy_calibration:
+        w = WORKLOADS[args.workload]
+        dtype = np.complex128 if w["dtype"] == "c128" else np.complex64
+        src, dst = hq.State(w["qubits"], dtype), hq.State(w["qubits"], dtype)
+        src.set_basis(0)
+        dst.copy_from(src)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            dst.copy_from(src)
+        wall = time.perf_counter() - t0
+        nbytes = (1 << w["qubits"]) * np.dtype(dtype).itemsize
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "steps": args.steps,
+                          "ms_per_copy": wall * 1e3 / args.steps,
+                          "copy_GBps": 2 * nbytes * args.steps / wall / 1e9}))
+        return
 
     w = WORKLOADS[args.workload]
     tun = dict(w["tuning"])
@@ -144,6 +170,8 @@ def main():
         tun["planner"] = args.planner
     if args.ops_block is not None:
         tun["ops_block"] = args.ops_block
+    if args.kernel is not None:
+        tun["kernel"] = args.kernel
     tuning = N.default_tuning(**tun)
     n = w["qubits"]
     dtype = np.complex128 if w["dtype"] == "c128" else np.complex64
@@ -191,7 +219,8 @@ def main():
                    else "window", "state_norm_after": norm},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_pass_lds", "bytes_per_launch": bytes_per_launch,
+                     "kernel": "k_pass_reg" if plan.kernel == N.HGM_KERNEL_REG else "k_pass_lds",
+                     "bytes_per_launch": bytes_per_launch,
                      "mean_launch_ms": mean_launch_ms},
     }
     if not args.no_cpu_baseline:

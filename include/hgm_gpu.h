@@ -143,6 +143,8 @@ void *hgm_state_device_ptr(hgm_state *s);
 /* |index> on the device, no host copy (setMemoryValue, src/core/dgm_core.cpp:63-65). */
 int hgm_state_set_basis(hgm_state *s, uint64_t index);
 int hgm_state_upload(hgm_state *s, const void *host);
+/* Device-to-device copy between states of equal size and dtype (one streaming kernel). */
+int hgm_state_copy(hgm_state *dst, const hgm_state *src);
 int hgm_state_download(const hgm_state *s, void *host);
 /* Apply a gate list `iterations` times (plans, uploads the op tables once, launches). */
 int hgm_state_apply(hgm_state *s, const hgm_op *ops, int64_t n_ops, const hgm_tuning *tuning,
@@ -170,6 +172,16 @@ int hgm_plan_pass(const hgm_plan *p, int64_t i, uint64_t *local_mask, int64_t *n
 int hgm_plan_shape(const hgm_plan *p, int32_t *region_bits, int32_t *block_size, int32_t *repeat_count,
                    int32_t *coalesced_bits);
 int hgm_state_run_plan(hgm_state *s, const hgm_plan *p, hgm_stats *stats);
+/* Kernel family the plan runs on (HGM_KERNEL_LDS or HGM_KERNEL_REG). */
+int hgm_plan_kernel(const hgm_plan *p);
+/* Register-kernel program of pass i, for inspection: rops receives n_rops
+ * records of 104 bytes, local_pos the 16 tile-bit -> state-bit entries, and
+ * first/last the phase maps of the load and store assignments. Passing NULL
+ * buffers queries n_rops. */
+int hgm_plan_program(const hgm_plan *p, int64_t i, void *rops, int64_t *n_rops, int32_t *first_map,
+                     int32_t *last_map, int8_t *local_pos);
+/* All phase maps of a register-kernel plan (records of 256 bytes). */
+int hgm_plan_maps(const hgm_plan *p, void *maps, int64_t *n_maps);
 
 /* Tuning check with the reference's rules plus the gfx950 limits; returns
  * HGM_E_TUNING with a message when the tuple is rejected. */

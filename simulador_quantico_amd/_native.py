@@ -73,6 +73,7 @@ SIGNATURES = [
     ("hgm_state_set_basis", C.c_int, [_P, C.c_uint64]),
     ("hgm_state_upload", C.c_int, [_P, _P]),
     ("hgm_state_download", C.c_int, [_P, _P]),
+    ("hgm_state_copy", C.c_int, [_P, _P]),
     ("hgm_state_apply", C.c_int, [_P, C.POINTER(Op), C.c_int64, C.POINTER(Tuning), C.c_int, C.POINTER(Stats)]),
     ("hgm_state_norm2", C.c_int, [_P, C.POINTER(C.c_double)]),
     ("hgm_state_prob_one", C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
@@ -86,6 +87,10 @@ SIGNATURES = [
     ("hgm_plan_shape", C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32)]),
     ("hgm_state_run_plan", C.c_int, [_P, _P, C.POINTER(Stats)]),
+    ("hgm_plan_kernel", C.c_int, [_P]),
+    ("hgm_plan_program", C.c_int, [_P, C.c_int64, _P, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32), _P]),
+    ("hgm_plan_maps", C.c_int, [_P, _P, C.POINTER(C.c_int64)]),
     ("hgm_validate_tuning", C.c_int, [C.c_int, C.c_int, C.POINTER(Tuning)]),
     ("hgm_dgm_create", C.c_int, [C.c_int, C.POINTER(_P)]),
     ("hgm_dgm_destroy", None, [_P]),

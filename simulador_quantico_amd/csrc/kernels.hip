@@ -134,6 +134,206 @@ __global__ void __launch_bounds__(1024) k_pass_lds(typename Vec<T>::type *__rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_pass_reg: the register kernel. Each thread keeps A amplitudes of its tile
+// in registers for the whole pass. Gates act on register pairs only (the
+// planner's second level schedules them in phases); between phases the tile
+// makes one round trip through LDS into the next phase's bit assignment.
+// HBM traffic per pass: one read and one write of every amplitude.
+//
+// Every per-gate condition is resolved by selecting the operator, never by
+// branching per amplitude: a register-bit condition picks the matrix or the
+// identity per register pair (tile-uniform, scalar selects), a thread-bit
+// condition picks it per lane once per gate; the updates themselves are
+// straight-line FMA code.
+
+template <class T> struct Mat {
+    T m0r, m0i, m1r, m1i, m2r, m2i, m3r, m3i;
+};
+
+template <class T>
+__device__ __forceinline__ Mat<T> load_mat(const RegOp &o) {
+    return {(T)o.m[0], (T)o.m[1], (T)o.m[2], (T)o.m[3], (T)o.m[4], (T)o.m[5], (T)o.m[6], (T)o.m[7]};
+}
+
+template <class T>
+__device__ __forceinline__ Mat<T> or_identity(const Mat<T> &m, bool id) {
+    return {id ? T(1) : m.m0r, id ? T(0) : m.m0i, id ? T(0) : m.m1r, id ? T(0) : m.m1i,
+            id ? T(0) : m.m2r, id ? T(0) : m.m2i, id ? T(1) : m.m3r, id ? T(0) : m.m3i};
+}
+
+template <class T, int A, int BIT, bool COND>
+__device__ __forceinline__ void reg_dense(typename Vec<T>::type (&a)[A], const Mat<T> &m, uint32_t rmask,
+                                          uint32_t rval) {
+    using V = typename Vec<T>::type;
+#pragma unroll
+    for (int k0 = 0; k0 < A; ++k0) {
+        if (k0 & (1 << BIT)) continue;
+        const int k1 = k0 | (1 << BIT);
+        const Mat<T> u = COND ? or_identity(m, ((uint32_t)k0 & rmask) != rval) : m;
+        const V x = a[k0], y = a[k1];
+        a[k0] = cmul2<V, T>(u.m0r, u.m0i, x, u.m1r, u.m1i, y);
+        a[k1] = cmul2<V, T>(u.m2r, u.m2i, x, u.m3r, u.m3i, y);
+    }
+}
+
+template <class T, int A, int BIT>
+__device__ __forceinline__ void reg_anti(typename Vec<T>::type (&a)[A], const Mat<T> &m) {
+    using V = typename Vec<T>::type;
+#pragma unroll
+    for (int k0 = 0; k0 < A; ++k0) {
+        if (k0 & (1 << BIT)) continue;
+        const int k1 = k0 | (1 << BIT);
+        const V x = a[k0], y = a[k1];
+        a[k0] = cmul<V, T>(m.m1r, m.m1i, y);
+        a[k1] = cmul<V, T>(m.m2r, m.m2i, x);
+    }
+}
+
+template <class T, int A, int BIT>
+__device__ __forceinline__ void reg_swap(typename Vec<T>::type (&a)[A]) {
+    using V = typename Vec<T>::type;
+#pragma unroll
+    for (int k0 = 0; k0 < A; ++k0) {
+        if (k0 & (1 << BIT)) continue;
+        const int k1 = k0 | (1 << BIT);
+        const V x = a[k0];
+        a[k0] = a[k1];
+        a[k1] = x;
+    }
+}
+
+// dispatch on the runtime register bit (tile-uniform) to unrolled code
+#define HGM_REG_BITS(CALL)                                  \
+    switch (bit) {                                          \
+        case 0: CALL(0); break;                             \
+        case 1: if constexpr (A > 2) { CALL(1); } break;    \
+        case 2: if constexpr (A > 4) { CALL(2); } break;    \
+        case 3: if constexpr (A > 8) { CALL(3); } break;    \
+        case 4: if constexpr (A > 16) { CALL(4); } break;   \
+        default: break;                                     \
+    }
+
+template <class T, int A>
+__device__ __forceinline__ void reg_gate(typename Vec<T>::type (&a)[A], const RegOp &o, int t) {
+    const int bit = o.rbit;
+    if (o.kind == R_SWAP) {
+#define HGM_CALL(B) reg_swap<T, A, B>(a)
+        HGM_REG_BITS(HGM_CALL)
+#undef HGM_CALL
+        return;
+    }
+    Mat<T> m = load_mat<T>(o);
+    if (o.kind == R_ANTI) {
+#define HGM_CALL(B) reg_anti<T, A, B>(a, m)
+        HGM_REG_BITS(HGM_CALL)
+#undef HGM_CALL
+        return;
+    }
+    if (o.tmask == 0) {   // tile-uniform matrix (scalar registers)
+        if (o.rmask) {
+#define HGM_CALL(B) reg_dense<T, A, B, true>(a, m, o.rmask, o.rval)
+            HGM_REG_BITS(HGM_CALL)
+#undef HGM_CALL
+        } else {
+#define HGM_CALL(B) reg_dense<T, A, B, false>(a, m, 0u, 0u)
+            HGM_REG_BITS(HGM_CALL)
+#undef HGM_CALL
+        }
+        return;
+    }
+    m = or_identity(m, ((uint32_t)t & o.tmask) != o.tval);   // per lane, once per gate
+#define HGM_CALL(B) reg_dense<T, A, B, true>(a, m, o.rmask, o.rval)
+    HGM_REG_BITS(HGM_CALL)
+#undef HGM_CALL
+}
+
+template <class T, int A>
+__device__ __forceinline__ void reg_phase_apply(typename Vec<T>::type (&a)[A], T fr, T fi, bool real,
+                                                uint32_t rmask, uint32_t rval) {
+    if (real) {
+#pragma unroll
+        for (int k = 0; k < A; ++k) {
+            const T fk = (((uint32_t)k & rmask) == rval) ? fr : T(1);
+            a[k].x *= fk;
+            a[k].y *= fk;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < A; ++k) {
+            const bool hit = ((uint32_t)k & rmask) == rval;
+            a[k] = cmul<typename Vec<T>::type, T>(hit ? fr : T(1), hit ? fi : T(0), a[k]);
+        }
+    }
+}
+
+template <class T, int A>
+__device__ __forceinline__ void reg_phase(typename Vec<T>::type (&a)[A], const RegOp &o, int t) {
+    const bool real = o.kind == R_PHASE_REAL;
+    if (o.tmask == 0) {   // tile-uniform factor (scalar registers)
+        reg_phase_apply<T, A>(a, (T)o.m[0], (T)o.m[1], real, o.rmask, o.rval);
+        return;
+    }
+    const bool lane_ok = ((uint32_t)t & o.tmask) == o.tval;
+    reg_phase_apply<T, A>(a, lane_ok ? (T)o.m[0] : T(1), lane_ok ? (T)o.m[1] : T(0), real, o.rmask, o.rval);
+}
+
+// Moves the tile into another bit assignment through LDS. The maps hold
+// swizzled tile indices (the swizzle is linear, so thread and register parts
+// combine with XOR).
+template <class T, int A>
+__device__ __forceinline__ void reg_remap(typename Vec<T>::type (&a)[A], typename Vec<T>::type *lds,
+                                          const PhaseMap &from, const PhaseMap &to, int t) {
+    __syncthreads();   // the previous users of the LDS tile are done
+    const uint32_t tf = (uint32_t)from.thr[0][t & 31] ^ (uint32_t)from.thr[1][(t >> 5) & 31];
+#pragma unroll
+    for (int k = 0; k < A; ++k) lds[tf ^ (uint32_t)from.reg[k]] = a[k];
+    __syncthreads();
+    const uint32_t tt = (uint32_t)to.thr[0][t & 31] ^ (uint32_t)to.thr[1][(t >> 5) & 31];
+#pragma unroll
+    for (int k = 0; k < A; ++k) a[k] = lds[tt ^ (uint32_t)to.reg[k]];
+}
+
+template <class T, int A>
+__global__ void __launch_bounds__(reg_kernel_max_threads(A)) k_pass_reg(typename Vec<T>::type *__restrict__ state,
+                                                   const RegOp *__restrict__ ops, int n_ops,
+                                                   const PhaseMap *__restrict__ maps, const PassMem *__restrict__ pm,
+                                                   uint64_t nonlocal_mask, int64_t n_tiles, int r) {
+    using V = typename Vec<T>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    V *lds = reinterpret_cast<V *>(smem);
+    const int t = threadIdx.x;
+    (void)r;
+    const uint64_t lt = pm->lthr[0][t & 31] | pm->lthr[1][(t >> 5) & 31];
+    const uint64_t st = pm->sthr[0][t & 31] | pm->sthr[1][(t >> 5) & 31];
+    for (int64_t tid = blockIdx.x; tid < n_tiles; tid += gridDim.x) {
+        const uint64_t base = pdep64((uint64_t)tid, nonlocal_mask);
+        V a[A];
+#pragma unroll
+        for (int k = 0; k < A; ++k) a[k] = state[base | lt | pm->lreg[k]];
+        int cur = pm->first_map;
+        for (int i = 0; i < n_ops; ++i) {
+            const RegOp &o = ops[i];
+            if (o.kind == R_REMAP) {
+                reg_remap<T, A>(a, lds, maps[cur], maps[o.rbit], t);
+                cur = o.rbit;
+                continue;
+            }
+            if ((base & o.gmask) != o.gval) continue;   // tile-uniform
+            if (o.kind == R_PHASE || o.kind == R_PHASE_REAL) reg_phase<T, A>(a, o, t);
+            else reg_gate<T, A>(a, o, t);
+        }
+#pragma unroll
+        for (int k = 0; k < A; ++k) state[base | st | pm->sreg[k]] = a[k];
+    }
+}
+
+// Plain streaming copy, 16 bytes per lane per iteration.
+__global__ void __launch_bounds__(256) k_copy(uint4 *__restrict__ dst, const uint4 *__restrict__ src, uint64_t n16) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 template <class T>
 __global__ void k_set_basis(typename Vec<T>::type *state, uint64_t n, uint64_t index) {
     using V = typename Vec<T>::type;
@@ -230,10 +430,54 @@ hipError_t pass_lds(void *state, const DevOp *ops, int n_ops, const PassArgs &a,
     return hipGetLastError();
 }
 
+template <class T, int A>
+hipError_t pass_reg(void *state, const RegOp *ops, int n_ops, const PhaseMap *maps, const PassMem *pm,
+                    uint64_t nonlocal, int64_t n_tiles, int r, int B, size_t lds, hipStream_t s) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipFuncSetAttribute((const void *)k_pass_reg<T, A>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_set = true;
+    }
+    int64_t grid = n_tiles < (int64_t(1) << 22) ? n_tiles : (int64_t(1) << 22);
+    hipLaunchKernelGGL((k_pass_reg<T, A>), dim3((unsigned)grid), dim3(B), lds, s,
+                       reinterpret_cast<typename Vec<T>::type *>(state), ops, n_ops, maps, pm, nonlocal, n_tiles, r);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t pass_reg_any(int A, void *state, const RegOp *ops, int n_ops, const PhaseMap *maps, const PassMem *pm,
+                        uint64_t nonlocal, int64_t n_tiles, int r, int B, size_t lds, hipStream_t s) {
+    switch (A) {
+        case 2: return pass_reg<T, 2>(state, ops, n_ops, maps, pm, nonlocal, n_tiles, r, B, lds, s);
+        case 4: return pass_reg<T, 4>(state, ops, n_ops, maps, pm, nonlocal, n_tiles, r, B, lds, s);
+        case 8: return pass_reg<T, 8>(state, ops, n_ops, maps, pm, nonlocal, n_tiles, r, B, lds, s);
+        case 16: return pass_reg<T, 16>(state, ops, n_ops, maps, pm, nonlocal, n_tiles, r, B, lds, s);
+        case 32: return pass_reg<T, 32>(state, ops, n_ops, maps, pm, nonlocal, n_tiles, r, B, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_pass_reg(int dtype, int A, void *state, const RegOp *ops, int n_ops, const PhaseMap *maps,
+                           const PassMem *pm, uint64_t nonlocal, int64_t n_tiles, int r, int B, size_t lds_bytes,
+                           hipStream_t s) {
+    return dtype == HGM_C64 ? pass_reg_any<float>(A, state, ops, n_ops, maps, pm, nonlocal, n_tiles, r, B, lds_bytes, s)
+                            : pass_reg_any<double>(A, state, ops, n_ops, maps, pm, nonlocal, n_tiles, r, B, lds_bytes, s);
+}
 
 hipError_t launch_pass_lds(int dtype, void *state, const DevOp *ops, int n_ops, const PassArgs &a, hipStream_t s) {
     return dtype == HGM_C64 ? pass_lds<float>(state, ops, n_ops, a, s) : pass_lds<double>(state, ops, n_ops, a, s);
+}
+
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    const uint64_t n16 = bytes / 16;
+    const int blk = 256;
+    uint64_t g = (n16 + blk - 1) / blk;
+    if (g > 8192) g = 8192;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)g), dim3(blk), 0, s, (uint4 *)dst, (const uint4 *)src, n16);
+    return hipGetLastError();
 }
 
 hipError_t launch_set_basis(int dtype, void *state, uint64_t n, uint64_t index, hipStream_t s) {

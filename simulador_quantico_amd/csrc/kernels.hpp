@@ -20,6 +20,15 @@ struct PassArgs {
 // the pass's gates are applied there, and the tile is written back.
 hipError_t launch_pass_lds(int dtype, void *state, const DevOp *ops, int n_ops, const PassArgs &a, hipStream_t s);
 
+// One full-state pass of the register kernel (A amplitudes per thread, B
+// threads per tile of 2^r); lds_bytes is 0 for a pass without remaps.
+hipError_t launch_pass_reg(int dtype, int A, void *state, const RegOp *ops, int n_ops, const PhaseMap *maps,
+                           const PassMem *pm, uint64_t nonlocal, int64_t n_tiles, int r, int B, size_t lds_bytes,
+                           hipStream_t s);
+
+// dst[0..bytes) = src[0..bytes) (bytes a multiple of 16).
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
+
 // state[i] = (i == index) for i < n_amps.
 hipError_t launch_set_basis(int dtype, void *state, uint64_t n_amps, uint64_t index, hipStream_t s);
 

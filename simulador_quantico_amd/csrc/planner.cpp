@@ -255,9 +255,233 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
             plan.passes.push_back(p);
         }
     }
+    const int A = 2 * plan.shape.R;
+    const bool reg_ok = A <= kMaxRegAmps && plan.shape.B <= reg_kernel_max_threads(A);
+    if (t.kernel == HGM_KERNEL_REG && !reg_ok) {
+        *err = "Erro de tuning: the register kernel holds at most " + std::to_string(kMaxRegAmps) +
+               " amplitudes per thread and at most " + std::to_string(reg_kernel_max_threads(A)) +
+               " threads for 2*repeat_count = " + std::to_string(A);
+        return HGM_E_TUNING;
+    }
+    if (t.kernel == HGM_KERNEL_REG || (t.kernel == HGM_KERNEL_AUTO && reg_ok)) {
+        plan.kernel = HGM_KERNEL_REG;
+        build_register_program(plan);
+    }
     plan.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = std::move(plan);
     return HGM_OK;
+}
+
+namespace {
+
+int popc(uint64_t v) { return __builtin_popcountll(v); }
+
+// Splits a tile-coordinate condition into register-index and thread-index parts.
+void split_cond(uint32_t mask, uint32_t val, uint32_t S, uint32_t all, RegOp &o) {
+    for (int b = 0; b < 32; ++b) {
+        if (!((mask >> b) & 1)) continue;
+        const uint32_t v = (val >> b) & 1;
+        const uint32_t below = (1u << b) - 1;
+        if ((S >> b) & 1) {
+            const int i = popc(S & below);
+            o.rmask |= 1u << i;
+            o.rval |= v << i;
+        } else {
+            const int j = popc(~S & all & below);
+            o.tmask |= 1u << j;
+            o.tval |= v << j;
+        }
+    }
+}
+
+// LDS slot of tile index x: the upper nibbles are XOR-folded into the low
+// one, so that lanes walking any 4 consecutive tile bits hit distinct 16-byte
+// slots. The map is linear over GF(2) and its own inverse.
+uint32_t lds_swizzle(uint32_t x) { return x ^ ((x >> 4) & 15u) ^ ((x >> 8) & 15u) ^ ((x >> 12) & 15u); }
+
+// Tile indices of a phase assignment (unswizzled).
+PhaseMap make_map(uint32_t S, int r) {
+    PhaseMap pm;
+    std::memset(&pm, 0, sizeof(pm));
+    int sb[kMaxTileBits], tbits[kMaxTileBits];
+    int na = 0, nt = 0;
+    for (int b = 0; b < r; ++b) {
+        if ((S >> b) & 1) sb[na++] = b;
+        else tbits[nt++] = b;
+    }
+    for (int k = 0; k < 64; ++k) {
+        uint32_t v = 0;
+        for (int i = 0; i < na; ++i)
+            if ((k >> i) & 1) v |= 1u << sb[i];
+        pm.reg[k] = (k < (1 << na)) ? (uint16_t)v : 0;
+    }
+    for (int h = 0; h < 2; ++h)
+        for (int x = 0; x < 32; ++x) {
+            uint32_t v = 0;
+            for (int j = 0; j < 5; ++j)
+                if (((x >> j) & 1) && 5 * h + j < nt) v |= 1u << tbits[5 * h + j];
+            pm.thr[h][x] = (uint16_t)v;
+        }
+    return pm;
+}
+
+PhaseMap swizzled(const PhaseMap &m) {
+    PhaseMap s;
+    for (int k = 0; k < 64; ++k) s.reg[k] = (uint16_t)lds_swizzle(m.reg[k]);
+    for (int h = 0; h < 2; ++h)
+        for (int x = 0; x < 32; ++x) s.thr[h][x] = (uint16_t)lds_swizzle(m.thr[h][x]);
+    return s;
+}
+
+uint64_t to_state(uint32_t tile_mask, const Pass &ps) {
+    uint64_t g = 0;
+    for (int b = 0; b < kMaxTileBits; ++b)
+        if ((tile_mask >> b) & 1) g |= uint64_t(1) << ps.local_pos[b];
+    return g;
+}
+
+bool is_one(double re, double im) { return re == 1.0 && im == 0.0; }
+
+}  // namespace
+
+// Second-level schedule: inside each pass, the same commutation-aware greedy
+// as the pass planner, with the tile as the "state" and the log2(A) register
+// bits as the "tile". The first and the last phase keep the coalesced tile
+// bits on the thread index so that HBM loads and stores stay contiguous
+// across lanes (an empty phase is added when a gate needs them earlier or the
+// schedule ends with them in registers).
+void build_register_program(Plan &plan) {
+    const int r = plan.shape.r, c = plan.shape.c;
+    const int A = 2 * plan.shape.R;
+    int a = 0;
+    while ((1 << a) < A) ++a;
+    const uint32_t all = (uint32_t)((1u << r) - 1);
+    const uint32_t forbid = (r - c >= a) ? (uint32_t)((1u << c) - 1) : 0u;
+    plan.rops.clear();
+    plan.maps.clear();
+    plan.mems.clear();
+    plan.max_phases = 0;
+
+    for (Pass &ps : plan.passes) {
+        const DevOp *ops = plan.ops.data() + ps.op_begin;
+        const int64_t n = ps.op_count;
+        std::vector<uint32_t> need(n), wr(n), rd(n);
+        for (int64_t i = 0; i < n; ++i) {
+            const DevOp &o = ops[i];
+            if (o.kind == K_DENSE || o.kind == K_ANTI) {
+                need[i] = wr[i] = 1u << o.tbit;
+                rd[i] = o.lmask;
+            } else if (o.kind == K_DIAG) {
+                need[i] = wr[i] = 0;
+                rd[i] = (1u << o.tbit) | o.lmask;
+            } else {
+                need[i] = wr[i] = 0;
+                rd[i] = o.lmask;
+            }
+        }
+        std::vector<uint32_t> sets;
+        std::vector<std::vector<int64_t>> lists;
+        std::vector<char> done((size_t)n, 0);
+        int64_t head = 0;
+        while (head < n) {
+            const bool first = sets.empty();
+            uint32_t S = 0, bw = 0, br = 0;
+            int nS = 0;
+            std::vector<int64_t> list;
+            for (int64_t k = head; k < n; ++k) {
+                if (done[k]) continue;
+                bool ok = !(wr[k] & (bw | br)) && !(rd[k] & bw);
+                if (ok && (need[k] & ~S)) {
+                    if (nS < a && !(first && (need[k] & forbid))) { S |= need[k]; ++nS; }
+                    else ok = false;
+                }
+                if (ok) { list.push_back(k); done[k] = 1; }
+                else { bw |= wr[k]; br |= rd[k]; if (bw == all) break; }
+            }
+            while (head < n && done[head]) ++head;
+            sets.push_back(S);
+            lists.push_back(std::move(list));
+        }
+        if (sets.empty() || (sets.back() & forbid)) { sets.push_back(0); lists.emplace_back(); }
+        // complete every assignment to a register bits, highest free bits first
+        // (never the coalesced ones in the first and last phase)
+        for (size_t i = 0; i < sets.size(); ++i) {
+            const bool edge = (i == 0 || i + 1 == sets.size());
+            for (int b = r - 1; b >= 0 && popc(sets[i]) < a; --b)
+                if (!((sets[i] >> b) & 1) && !(edge && ((forbid >> b) & 1))) sets[i] |= 1u << b;
+            for (int b = r - 1; b >= 0 && popc(sets[i]) < a; --b) sets[i] |= 1u << b;
+        }
+        const int32_t map0 = (int32_t)plan.maps.size();
+        const PhaseMap first_map = make_map(sets.front(), r), last_map = make_map(sets.back(), r);
+        for (uint32_t S : sets) plan.maps.push_back(swizzled(make_map(S, r)));
+
+        ps.rop_begin = (int64_t)plan.rops.size();
+        for (size_t i = 0; i < sets.size(); ++i) {
+            const uint32_t S = sets[i];
+            if (i > 0) {
+                RegOp o;
+                std::memset(&o, 0, sizeof(o));
+                o.kind = R_REMAP;
+                o.rbit = map0 + (int32_t)i;
+                plan.rops.push_back(o);
+            }
+            for (int64_t k : lists[i]) {
+                const DevOp &d = ops[k];
+                if (d.kind == K_DENSE || d.kind == K_ANTI) {
+                    RegOp o;
+                    std::memset(&o, 0, sizeof(o));
+                    o.kind = d.kind == K_DENSE ? R_DENSE : R_ANTI;
+                    if (d.kind == K_ANTI && is_one(d.m[2], d.m[3]) && is_one(d.m[4], d.m[5])) o.kind = R_SWAP;
+                    o.rbit = popc(S & ((1u << d.tbit) - 1));
+                    split_cond(d.lmask, d.lval, S, all, o);
+                    // conditioned anti-diagonal gates run as dense ones: the
+                    // kernel resolves conditions by selecting matrix or identity
+                    if (o.kind != R_DENSE && (o.rmask || o.tmask)) o.kind = R_DENSE;
+                    o.gmask = d.gmask;
+                    o.gval = d.gval;
+                    std::memcpy(o.m, d.m, sizeof(o.m));
+                    plan.rops.push_back(o);
+                    continue;
+                }
+                for (int v = 0; v < 2; ++v) {   // diagonal: one conditional phase per target value
+                    const double fr = d.m[6 * v], fi = d.m[6 * v + 1];
+                    if (is_one(fr, fi)) continue;
+                    RegOp o;
+                    std::memset(&o, 0, sizeof(o));
+                    o.kind = fi == 0.0 ? R_PHASE_REAL : R_PHASE;
+                    uint32_t lm = d.lmask, lv = d.lval;
+                    o.gmask = d.gmask;
+                    o.gval = d.gval;
+                    if (d.kind == K_DIAG) { lm |= 1u << d.tbit; lv |= uint32_t(v) << d.tbit; }
+                    else { o.gmask |= uint64_t(1) << d.tbit; o.gval |= uint64_t(v) << d.tbit; }
+                    split_cond(lm, lv, S, all, o);
+                    o.m[0] = fr;
+                    o.m[1] = fi;
+                    plan.rops.push_back(o);
+                }
+            }
+        }
+        ps.rop_count = (int64_t)plan.rops.size() - ps.rop_begin;
+        ps.phases = (int32_t)sets.size();
+        if (ps.phases > plan.max_phases) plan.max_phases = ps.phases;
+
+        PassMem pm;
+        std::memset(&pm, 0, sizeof(pm));
+        const PhaseMap &f = first_map, &l = last_map;
+        for (int h = 0; h < 2; ++h)
+            for (int x = 0; x < 32; ++x) {
+                pm.lthr[h][x] = to_state(f.thr[h][x], ps);
+                pm.sthr[h][x] = to_state(l.thr[h][x], ps);
+            }
+        for (int k = 0; k < 64; ++k) {
+            pm.lreg[k] = to_state(f.reg[k], ps);
+            pm.sreg[k] = to_state(l.reg[k], ps);
+        }
+        pm.first_map = map0;
+        pm.last_map = (int32_t)plan.maps.size() - 1;
+        ps.mem = (int64_t)plan.mems.size();
+        plan.mems.push_back(pm);
+    }
 }
 
 }  // namespace hgm

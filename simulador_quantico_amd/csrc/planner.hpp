@@ -45,6 +45,13 @@ struct DevOp {
 static_assert(sizeof(DevOp) == 96, "DevOp layout is shared with the kernels");
 
 constexpr int kMaxTileBits = 16;
+constexpr int kMaxRegAmps = 32;      // amplitudes per thread of the register kernel
+constexpr int kMaxThreadBits = 10;   // 1024 threads per workgroup
+
+// Workgroup size limit of the register kernel for A amplitudes per thread:
+// the VGPR budget per lane is 512 / (waves per SIMD), and A complex doubles
+// take 4A VGPRs.
+constexpr int reg_kernel_max_threads(int A) { return A >= 32 ? 256 : (A >= 16 ? 512 : 1024); }
 
 struct TileShape {
     int r;      // tile bits
@@ -58,7 +65,60 @@ struct Pass {
     int8_t local_pos[kMaxTileBits];   // tile bit j -> state-index bit
     int64_t op_begin;                 // first DevOp of the pass
     int64_t op_count;
+    int64_t rop_begin = 0;            // register program of the pass (RegOp range)
+    int64_t rop_count = 0;
+    int64_t mem = -1;                 // PassMem index of the register program
+    int32_t phases = 0;               // register phases (LDS remaps = phases - 1)
 };
+
+// ---- register kernel program --------------------------------------------
+// In the register kernel each thread holds A = 2R amplitudes of its tile in
+// registers. A "phase" assigns log2(A) tile bits to the register index and the
+// other tile bits, in ascending order, to the thread index. Gates are applied
+// to registers only; a remap moves the tile through LDS into the next phase's
+// assignment.
+enum RegKind : int32_t {
+    R_DENSE = 0,       // 2x2 update of the register pair differing in register bit `rbit`
+    R_ANTI = 1,        // swap-and-scale of that pair (m00 == m11 == 0)
+    R_PHASE = 2,       // multiply by (m[0], m[1]) where all three conditions hold
+    R_REMAP = 3,       // move to phase map `rbit` through LDS
+    R_SWAP = 4,        // exchange of the pair (the X matrix)
+    R_PHASE_REAL = 5   // R_PHASE with a real factor m[0] (m[1] == 0)
+};
+
+struct RegOp {
+    int32_t kind;
+    int32_t rbit;       // register-index bit of the target, or the next PhaseMap (R_REMAP)
+    uint32_t rmask;     // condition on register-index bits
+    uint32_t rval;
+    uint32_t tmask;     // condition on thread-index bits
+    uint32_t tval;
+    uint64_t gmask;     // condition on non-local state-index bits
+    uint64_t gval;
+    double m[8];
+};
+static_assert(sizeof(RegOp) == 104, "RegOp layout is shared with the kernels");
+
+// Tile index of the element a thread holds in register k:
+//   thr[0][t & 31] | thr[1][t >> 5] | reg[k]
+struct PhaseMap {
+    uint16_t reg[64];
+    uint16_t thr[2][32];
+};
+static_assert(sizeof(PhaseMap) == 256, "PhaseMap layout is shared with the kernels");
+
+// State-index offsets of the load (first phase) and store (last phase)
+// assignments: element (t, k) of tile `base` lives at
+//   base | lthr[0][t & 31] | lthr[1][t >> 5] | lreg[k]
+struct PassMem {
+    uint64_t lthr[2][32];
+    uint64_t lreg[64];
+    uint64_t sthr[2][32];
+    uint64_t sreg[64];
+    int32_t first_map;
+    int32_t last_map;
+};
+static_assert(sizeof(PassMem) == 2056, "PassMem layout is shared with the kernels");
 
 struct Plan {
     int qubits = 0;
@@ -70,7 +130,15 @@ struct Plan {
     std::vector<int64_t> src_index;   // index of each DevOp in the iteration-expanded gate list
     int64_t gates = 0;
     double plan_ms = 0;
+    // register kernel program (kernel == HGM_KERNEL_REG)
+    std::vector<RegOp> rops;
+    std::vector<PhaseMap> maps;
+    std::vector<PassMem> mems;
+    int max_phases = 0;
 };
+
+// Builds the register program of every pass (fills rops/maps/mems).
+void build_register_program(Plan &plan);
 
 size_t amp_bytes(int dtype);
 

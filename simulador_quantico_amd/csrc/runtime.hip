@@ -49,6 +49,12 @@ struct Ctx {
     int cus = 0;
     DevOp *ops = nullptr;       // op tables of the current call (grow-only)
     size_t ops_cap = 0;
+    void *rops = nullptr;       // register-kernel programs, phase maps, pass tables (grow-only)
+    size_t rops_cap = 0;
+    void *maps = nullptr;
+    size_t maps_cap = 0;
+    void *mems = nullptr;
+    size_t mems_cap = 0;
     void *cache = nullptr;      // device buffer of hgm_execute (grow-only)
     size_t cache_cap = 0;
     double *partial = nullptr;  // reduction partials (device)
@@ -118,6 +124,16 @@ int reserve_ops(size_t n) {
     return HGM_OK;
 }
 
+int reserve(void **buf, size_t *cap, size_t bytes, const char *what) {
+    Ctx &c = ctx();
+    if (bytes <= *cap) return HGM_OK;
+    if (*buf) { hipStreamSynchronize(c.stream); hipFree(*buf); *buf = nullptr; *cap = 0; }
+    size_t want = bytes < 4096 ? 4096 : bytes + bytes / 2;
+    HGM_HIP(hipMalloc(buf, want), what);
+    *cap = want;
+    return HGM_OK;
+}
+
 float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0;
     hipEventElapsedTime(&ms, a, b);
@@ -131,22 +147,47 @@ int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st) {
         t_err = "plan was built for another qubit count or dtype";
         return HGM_E_STATE;
     }
-    int rc = reserve_ops(p.ops.size());
-    if (rc) return rc;
-    if (!p.ops.empty())
-        HGM_HIP(hipMemcpyAsync(c.ops, p.ops.data(), p.ops.size() * sizeof(DevOp), hipMemcpyHostToDevice, c.stream),
-                "hipMemcpyAsync(op tables)");
+    const bool reg = p.kernel == HGM_KERNEL_REG;
+    int rc;
+    if (reg) {
+        if ((rc = reserve(&c.rops, &c.rops_cap, p.rops.size() * sizeof(RegOp), "hipMalloc(programs)"))) return rc;
+        if ((rc = reserve(&c.maps, &c.maps_cap, p.maps.size() * sizeof(PhaseMap), "hipMalloc(maps)"))) return rc;
+        if ((rc = reserve(&c.mems, &c.mems_cap, p.mems.size() * sizeof(PassMem), "hipMalloc(pass tables)"))) return rc;
+        if (!p.rops.empty())
+            HGM_HIP(hipMemcpyAsync(c.rops, p.rops.data(), p.rops.size() * sizeof(RegOp), hipMemcpyHostToDevice,
+                                   c.stream), "hipMemcpyAsync(programs)");
+        if (!p.maps.empty())
+            HGM_HIP(hipMemcpyAsync(c.maps, p.maps.data(), p.maps.size() * sizeof(PhaseMap), hipMemcpyHostToDevice,
+                                   c.stream), "hipMemcpyAsync(maps)");
+        if (!p.mems.empty())
+            HGM_HIP(hipMemcpyAsync(c.mems, p.mems.data(), p.mems.size() * sizeof(PassMem), hipMemcpyHostToDevice,
+                                   c.stream), "hipMemcpyAsync(pass tables)");
+    } else {
+        if ((rc = reserve_ops(p.ops.size()))) return rc;
+        if (!p.ops.empty())
+            HGM_HIP(hipMemcpyAsync(c.ops, p.ops.data(), p.ops.size() * sizeof(DevOp), hipMemcpyHostToDevice, c.stream),
+                    "hipMemcpyAsync(op tables)");
+    }
     PassArgs a;
     std::memset(&a, 0, sizeof(a));
     a.r = p.shape.r; a.c = p.shape.c; a.B = p.shape.B; a.R = p.shape.R;
     a.n_tiles = int64_t(1) << (p.qubits - p.shape.r);
     const uint64_t all = (uint64_t(1) << p.qubits) - 1;
+    const size_t tile_bytes = (size_t(1) << p.shape.r) * amp_bytes(p.dtype);
     HGM_HIP(hipEventRecord(c.e0, c.stream), "hipEventRecord");
     for (const Pass &ps : p.passes) {
         if (ps.op_count == 0) continue;
         a.nonlocal_mask = all & ~ps.local_mask;
-        std::memcpy(a.local_pos, ps.local_pos, sizeof(a.local_pos));
-        HGM_HIP(launch_pass_lds(p.dtype, s->d, c.ops + ps.op_begin, (int)ps.op_count, a, c.stream), "pass launch");
+        if (reg) {
+            HGM_HIP(launch_pass_reg(p.dtype, 2 * p.shape.R, s->d, static_cast<const RegOp *>(c.rops) + ps.rop_begin,
+                                    (int)ps.rop_count, static_cast<const PhaseMap *>(c.maps),
+                                    static_cast<const PassMem *>(c.mems) + ps.mem, a.nonlocal_mask, a.n_tiles,
+                                    p.shape.r, p.shape.B, ps.phases > 1 ? tile_bytes : 0, c.stream),
+                    "pass launch");
+        } else {
+            std::memcpy(a.local_pos, ps.local_pos, sizeof(a.local_pos));
+            HGM_HIP(launch_pass_lds(p.dtype, s->d, c.ops + ps.op_begin, (int)ps.op_count, a, c.stream), "pass launch");
+        }
     }
     HGM_HIP(hipEventRecord(c.e1, c.stream), "hipEventRecord");
     HGM_HIP(hipEventSynchronize(c.e1), "pass execution");
@@ -329,6 +370,18 @@ int hgm_state_set_basis(hgm_state *s, uint64_t index) {
     return HGM_OK;
 }
 
+int hgm_state_copy(hgm_state *dst, const hgm_state *src) {
+    if (!dst || !src || dst->bytes != src->bytes || dst->dtype != src->dtype) {
+        t_err = "hgm_state_copy: states differ in size or dtype";
+        return HGM_E_ARG;
+    }
+    Ctx &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    HGM_HIP(launch_copy(dst->d, src->d, src->bytes, c.stream), "copy");
+    HGM_HIP(hipStreamSynchronize(c.stream), "copy");
+    return HGM_OK;
+}
+
 int hgm_state_upload(hgm_state *s, const void *host) {
     if (!s || !host) { t_err = "null argument"; return HGM_E_ARG; }
     Ctx &c = ctx();
@@ -439,6 +492,28 @@ int hgm_plan_shape(const hgm_plan *p, int32_t *region_bits, int32_t *block_size,
     if (block_size) *block_size = p->plan.shape.B;
     if (repeat_count) *repeat_count = p->plan.shape.R;
     if (coalesced_bits) *coalesced_bits = p->plan.shape.c;
+    return HGM_OK;
+}
+
+int hgm_plan_kernel(const hgm_plan *p) { return p ? p->plan.kernel : -1; }
+
+int hgm_plan_program(const hgm_plan *p, int64_t i, void *rops, int64_t *n_rops, int32_t *first_map,
+                     int32_t *last_map, int8_t *local_pos) {
+    if (!p || i < 0 || i >= (int64_t)p->plan.passes.size() || !n_rops) { t_err = "bad arguments"; return HGM_E_ARG; }
+    const Pass &ps = p->plan.passes[i];
+    *n_rops = p->plan.kernel == HGM_KERNEL_REG ? ps.rop_count : 0;
+    if (rops && *n_rops)
+        std::memcpy(rops, p->plan.rops.data() + ps.rop_begin, (size_t)ps.rop_count * sizeof(RegOp));
+    if (first_map) *first_map = ps.mem >= 0 ? p->plan.mems[ps.mem].first_map : -1;
+    if (last_map) *last_map = ps.mem >= 0 ? p->plan.mems[ps.mem].last_map : -1;
+    if (local_pos) std::memcpy(local_pos, ps.local_pos, sizeof(ps.local_pos));
+    return HGM_OK;
+}
+
+int hgm_plan_maps(const hgm_plan *p, void *maps, int64_t *n_maps) {
+    if (!p || !n_maps) { t_err = "bad arguments"; return HGM_E_ARG; }
+    *n_maps = (int64_t)p->plan.maps.size();
+    if (maps && *n_maps) std::memcpy(maps, p->plan.maps.data(), p->plan.maps.size() * sizeof(PhaseMap));
     return HGM_OK;
 }
 

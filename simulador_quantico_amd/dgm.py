@@ -223,6 +223,9 @@ class State:
         assert a.size == 1 << self.qubits
         check(lib.hgm_state_upload(self._h, a.ctypes.data))
 
+    def copy_from(self, other: "State") -> None:
+        check(lib.hgm_state_copy(self._h, other._h))
+
     def download(self) -> np.ndarray:
         out = np.empty(1 << self.qubits, dtype=self.dtype)
         check(lib.hgm_state_download(self._h, out.ctypes.data))
@@ -295,3 +298,33 @@ class Plan:
         st = Stats()
         check(lib.hgm_state_run_plan(state._h, self._h, C.byref(st)))
         return st.as_dict()
+
+    @property
+    def kernel(self) -> int:
+        return int(lib.hgm_plan_kernel(self._h))
+
+    def program(self, i: int):
+        """Register-kernel program of pass i: (structured ops, first map, last map, local_pos)."""
+        n = C.c_int64()
+        f, l = C.c_int32(), C.c_int32()
+        check(lib.hgm_plan_program(self._h, i, None, C.byref(n), C.byref(f), C.byref(l), None))
+        ops = np.zeros(n.value, dtype=REGOP_DTYPE)
+        pos = np.zeros(16, dtype=np.int8)
+        check(lib.hgm_plan_program(self._h, i, ops.ctypes.data, C.byref(n), C.byref(f), C.byref(l),
+                                   pos.ctypes.data))
+        return ops, f.value, l.value, pos
+
+    def maps(self):
+        n = C.c_int64()
+        check(lib.hgm_plan_maps(self._h, None, C.byref(n)))
+        maps = np.zeros(n.value, dtype=PHASEMAP_DTYPE)
+        check(lib.hgm_plan_maps(self._h, maps.ctypes.data, C.byref(n)))
+        return maps
+
+
+# record layouts of csrc/planner.hpp (RegOp, PhaseMap)
+REGOP_DTYPE = np.dtype([("kind", np.int32), ("rbit", np.int32), ("rmask", np.uint32), ("rval", np.uint32),
+                        ("tmask", np.uint32), ("tval", np.uint32), ("gmask", np.uint64), ("gval", np.uint64),
+                        ("m", np.float64, 8)])
+PHASEMAP_DTYPE = np.dtype([("reg", np.uint16, 64), ("thr", np.uint16, (2, 32))])
+assert REGOP_DTYPE.itemsize == 104 and PHASEMAP_DTYPE.itemsize == 256
