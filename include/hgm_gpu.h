@@ -59,7 +59,10 @@ enum { HGM_PLAN_GREEDY = 0, /* dependency-aware lookahead, any set of tile-local
        HGM_PLAN_WINDOW = 1  /* in-order, contiguous window (the contract of src/core/kernel.cu:199-235) */ };
 
 /* kernel families */
-enum { HGM_KERNEL_AUTO = 0, HGM_KERNEL_LDS = 1, HGM_KERNEL_REG = 2 };
+enum { HGM_KERNEL_AUTO = 0,  /* the register kernel when the tile shape allows it, else LDS */
+       HGM_KERNEL_LDS = 1,   /* tile in LDS, one LDS round trip per gate */
+       HGM_KERNEL_REG = 2,   /* amplitudes in registers, interpreted per-pass program */
+       HGM_KERNEL_JIT = 3    /* the register kernel's programs compiled to straight-line code at plan time */ };
 
 /* One compiled single-qubit gate: the POD form of the reference's PT
  * (include/core/common.h:57-90). */
@@ -182,6 +185,12 @@ int hgm_plan_program(const hgm_plan *p, int64_t i, void *rops, int64_t *n_rops, 
                      int32_t *last_map, int8_t *local_pos);
 /* All phase maps of a register-kernel plan (records of 256 bytes). */
 int hgm_plan_maps(const hgm_plan *p, void *maps, int64_t *n_maps);
+/* Generated source of an HGM_KERNEL_JIT plan (empty for other kernels); out
+ * may be NULL to query the length. */
+int hgm_plan_jit_source(const hgm_plan *p, char *out, size_t cap, size_t *len);
+/* hipRTC compile time of an HGM_KERNEL_JIT plan in seconds (0 when the code
+ * came from the process cache or the plan is not JIT). */
+double hgm_plan_compile_seconds(const hgm_plan *p);
 
 /* Tuning check with the reference's rules plus the gfx950 limits; returns
  * HGM_E_TUNING with a message when the tuple is rejected. */

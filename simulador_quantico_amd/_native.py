@@ -20,6 +20,7 @@ HGM_PLAN_WINDOW = 1
 HGM_KERNEL_AUTO = 0
 HGM_KERNEL_LDS = 1
 HGM_KERNEL_REG = 2
+HGM_KERNEL_JIT = 3
 
 ERROR_NAMES = {1: "HGM_E_ARG", 2: "HGM_E_TUNING", 3: "HGM_E_HIP", 4: "HGM_E_NOMEM", 5: "HGM_E_PARSE",
                6: "HGM_E_STATE", 7: "HGM_E_UNSUPPORTED"}
@@ -91,6 +92,8 @@ SIGNATURES = [
     ("hgm_plan_program", C.c_int, [_P, C.c_int64, _P, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int32), _P]),
     ("hgm_plan_maps", C.c_int, [_P, _P, C.POINTER(C.c_int64)]),
+    ("hgm_plan_jit_source", C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("hgm_plan_compile_seconds", C.c_double, [_P]),
     ("hgm_validate_tuning", C.c_int, [C.c_int, C.c_int, C.POINTER(Tuning)]),
     ("hgm_dgm_create", C.c_int, [C.c_int, C.POINTER(_P)]),
     ("hgm_dgm_destroy", None, [_P]),

@@ -1,0 +1,319 @@
+// Plan-time specialisation of the register kernel (see jit.hpp).
+#include "jit.hpp"
+
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <unordered_map>
+
+namespace hgm {
+
+namespace {
+
+uint64_t fnv1a(const std::string &s) {
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char c : s) { h ^= c; h *= 1099511628211ull; }
+    return h;
+}
+
+std::string hex64(uint64_t v) {
+    char b[32];
+    std::snprintf(b, sizeof b, "0x%llxull", (unsigned long long)v);
+    return b;
+}
+
+std::string hex32(uint32_t v) {
+    char b[24];
+    std::snprintf(b, sizeof b, "0x%xu", v);
+    return b;
+}
+
+struct Gen {
+    bool f32;
+    std::string V, T;
+
+    double round(double v) const { return f32 ? (double)(float)v : v; }
+
+    std::string lit(double v) const {
+        char b[64];
+        if (f32) std::snprintf(b, sizeof b, "%af", (double)(float)v);
+        else std::snprintf(b, sizeof b, "%a", v);
+        return b;
+    }
+
+    // sum of coef * var, skipping zero coefficients and folding +-1
+    std::string lin(const std::vector<std::pair<double, std::string>> &terms) const {
+        std::string s;
+        for (const auto &tv : terms) {
+            const double c = round(tv.first);
+            if (c == 0.0) continue;
+            if (c == 1.0) s += (s.empty() ? "" : " + ") + tv.second;
+            else if (c == -1.0) s += (s.empty() ? "-" : " - ") + tv.second;
+            else s += (s.empty() ? "" : " + ") + lit(c) + " * " + tv.second;
+        }
+        return s.empty() ? lit(0.0) : s;
+    }
+};
+
+// Expression of a thread-dependent quantity built from single-bit
+// contributions: OR (state offsets) of ((t >> j) & 1) << bit_j, grouping runs.
+std::string thread_or(const uint64_t *contrib, int nbits) {
+    std::string s;
+    int j = 0;
+    while (j < nbits) {
+        if (contrib[j] == 0) { ++j; continue; }
+        int g = __builtin_ctzll(contrib[j]);
+        int len = 1;
+        while (j + len < nbits && contrib[j + len] == (uint64_t(1) << (g + len))) ++len;
+        char b[128];
+        std::snprintf(b, sizeof b, "(((unsigned long long)(t >> %d) & 0x%llxull) << %d)", j,
+                      (unsigned long long)((1ull << len) - 1), g);
+        s += (s.empty() ? "" : " | ") + std::string(b);
+        j += len;
+    }
+    return s.empty() ? "0ull" : s;
+}
+
+// Sum of ((t >> j) & 1) * contrib_j.
+std::string thread_sum(const uint32_t *contrib, int nbits) {
+    std::string s;
+    for (int j = 0; j < nbits; ++j) {
+        if (!contrib[j]) continue;
+        char b[96];
+        std::snprintf(b, sizeof b, "(((t >> %d) & 1u) * %uu)", j, contrib[j]);
+        s += (s.empty() ? "" : " + ") + std::string(b);
+    }
+    return s.empty() ? "0u" : s;
+}
+
+// The interpreter's phase maps hold XOR-swizzled tile indices; the swizzle is
+// its own inverse.
+uint32_t unswizzle(uint32_t x) { return x ^ ((x >> 4) & 15u) ^ ((x >> 8) & 15u) ^ ((x >> 12) & 15u); }
+
+// LDS slot of tile index x in the generated kernels: one padding slot per 16
+// and per 256 entries. Additive over disjoint bit sets, so a slot is a
+// per-thread part plus a per-register constant (an immediate offset).
+uint32_t padded(uint32_t x) { return x + (x >> 4) + (x >> 8); }
+
+// base = deposit of the tile id into the non-local state bits, by runs
+std::string base_expr(uint64_t nonlocal) {
+    std::string s;
+    int q = 0;
+    for (int p = 0; p < 64;) {
+        if (!((nonlocal >> p) & 1)) { ++p; continue; }
+        int len = 0;
+        while (p + len < 64 && ((nonlocal >> (p + len)) & 1)) ++len;
+        char b[128];
+        std::snprintf(b, sizeof b, "(((unsigned long long)tid >> %d) & 0x%llxull) << %d", q,
+                      (unsigned long long)(len >= 64 ? ~0ull : ((1ull << len) - 1)), p);
+        s += (s.empty() ? "" : " | ") + std::string("(") + b + ")";
+        q += len;
+        p += len;
+    }
+    return s.empty() ? "0ull" : s;
+}
+
+std::string reg(int k) { return "a" + std::to_string(k); }
+
+void emit_pair(const Gen &g, std::string &o, const RegOp &op, int k0, int k1) {
+    const double *m = op.m;
+    const std::string x = "x" , y = "y";
+    o += "        { const " + g.V + " x = " + reg(k0) + ", y = " + reg(k1) + ";\n";
+    if (op.kind == R_SWAP) {
+        o += "          " + reg(k0) + " = y; " + reg(k1) + " = x; }\n";
+        return;
+    }
+    if (op.kind == R_ANTI) {
+        o += "          " + reg(k0) + ".x = " + g.lin({{m[2], "y.x"}, {-m[3], "y.y"}}) + ";\n";
+        o += "          " + reg(k0) + ".y = " + g.lin({{m[2], "y.y"}, {m[3], "y.x"}}) + ";\n";
+        o += "          " + reg(k1) + ".x = " + g.lin({{m[4], "x.x"}, {-m[5], "x.y"}}) + ";\n";
+        o += "          " + reg(k1) + ".y = " + g.lin({{m[4], "x.y"}, {m[5], "x.x"}}) + "; }\n";
+        return;
+    }
+    o += "          " + reg(k0) + ".x = " +
+         g.lin({{m[0], "x.x"}, {-m[1], "x.y"}, {m[2], "y.x"}, {-m[3], "y.y"}}) + ";\n";
+    o += "          " + reg(k0) + ".y = " + g.lin({{m[0], "x.y"}, {m[1], "x.x"}, {m[2], "y.y"}, {m[3], "y.x"}}) + ";\n";
+    o += "          " + reg(k1) + ".x = " +
+         g.lin({{m[4], "x.x"}, {-m[5], "x.y"}, {m[6], "y.x"}, {-m[7], "y.y"}}) + ";\n";
+    o += "          " + reg(k1) + ".y = " + g.lin({{m[4], "x.y"}, {m[5], "x.x"}, {m[6], "y.y"}, {m[7], "y.x"}}) +
+         "; }\n";
+}
+
+void emit_phase(const Gen &g, std::string &o, const RegOp &op, int k) {
+    const double fr = g.round(op.m[0]), fi = g.round(op.m[1]);
+    const std::string a = reg(k);
+    if (fi == 0.0) {
+        if (fr == 1.0) return;
+        if (fr == -1.0) { o += "        " + a + ".x = -" + a + ".x; " + a + ".y = -" + a + ".y;\n"; return; }
+        o += "        " + a + ".x *= " + g.lit(fr) + "; " + a + ".y *= " + g.lit(fr) + ";\n";
+        return;
+    }
+    o += "        { const " + g.V + " x = " + a + ";\n";
+    o += "          " + a + ".x = " + g.lin({{fr, "x.x"}, {-fi, "x.y"}}) + ";\n";
+    o += "          " + a + ".y = " + g.lin({{fr, "x.y"}, {fi, "x.x"}}) + "; }\n";
+}
+
+std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const std::string &name) {
+    const int r = plan.shape.r, B = plan.shape.B, A = 2 * plan.shape.R;
+    int tb = 0;
+    while ((1 << tb) < B) ++tb;
+    const PassMem &pm = plan.mems[ps.mem];
+    const uint64_t all = (plan.qubits >= 64) ? ~0ull : ((1ull << plan.qubits) - 1);
+    const uint64_t nonlocal = all & ~ps.local_mask;
+    const int64_t n_tiles = int64_t(1) << (plan.qubits - r);
+
+    uint64_t lt[kMaxThreadBits] = {0}, st[kMaxThreadBits] = {0};
+    for (int j = 0; j < tb; ++j) {
+        lt[j] = pm.lthr[j / 5][1 << (j % 5)];
+        st[j] = pm.sthr[j / 5][1 << (j % 5)];
+    }
+    std::string o;
+    o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(B) + ") " + name + "(" + g.V +
+         " *__restrict__ s) {\n";
+    if (ps.phases > 1) o += "    extern __shared__ " + g.V + " lds[];\n";
+    o += "    const unsigned t = threadIdx.x;\n";
+    o += "    const unsigned long long lt = " + thread_or(lt, tb) + ";\n";
+    o += "    const unsigned long long st = " + thread_or(st, tb) + ";\n";
+    // thread parts of the LDS slot of every phase map of this pass
+    const int m_first = pm.first_map, m_last = pm.last_map;
+    if (ps.phases > 1)
+        for (int mi = m_first; mi <= m_last; ++mi) {
+            uint32_t c[kMaxThreadBits] = {0};
+            for (int j = 0; j < tb; ++j) c[j] = padded(unswizzle(plan.maps[mi].thr[j / 5][1 << (j % 5)]));
+            o += "    const unsigned f" + std::to_string(mi - m_first) + " = " + thread_sum(c, tb) + ";\n";
+        }
+    // one tile per workgroup: no loop, so the matrix literals are not kept
+    // live across tiles
+    (void)n_tiles;
+    o += "    {\n";
+    o += "        const unsigned long long tid = blockIdx.x;\n";
+    o += "        const unsigned long long base = " + base_expr(nonlocal) + ";\n";
+    for (int k = 0; k < A; ++k)
+        o += "        " + g.V + " " + reg(k) + " = s[(base | " + hex64(pm.lreg[k]) + ") + lt];\n";
+    int cur = m_first;
+    const RegOp *ops = plan.rops.data() + ps.rop_begin;
+    for (int64_t i = 0; i < ps.rop_count; ++i) {
+        const RegOp &op = ops[i];
+        if (op.kind == R_REMAP) {
+            const int nxt = op.rbit;
+            o += "        __syncthreads();\n";
+            for (int k = 0; k < A; ++k)
+                o += "        lds[f" + std::to_string(cur - m_first) + " + " +
+                     std::to_string(padded(unswizzle(plan.maps[cur].reg[k]))) + "u] = " + reg(k) + ";\n";
+            o += "        __syncthreads();\n";
+            for (int k = 0; k < A; ++k)
+                o += "        " + reg(k) + " = lds[f" + std::to_string(nxt - m_first) + " + " +
+                     std::to_string(padded(unswizzle(plan.maps[nxt].reg[k]))) + "u];\n";
+            cur = nxt;
+            continue;
+        }
+        std::string body;
+        if (op.kind == R_PHASE || op.kind == R_PHASE_REAL) {
+            for (int k = 0; k < A; ++k)
+                if (((uint32_t)k & op.rmask) == op.rval) emit_phase(g, body, op, k);
+        } else {
+            for (int k0 = 0; k0 < A; ++k0) {
+                if ((k0 >> op.rbit) & 1) continue;
+                if (((uint32_t)k0 & op.rmask) != op.rval) continue;
+                emit_pair(g, body, op, k0, k0 | (1 << op.rbit));
+            }
+        }
+        if (body.empty()) continue;
+        std::string cond;
+        if (op.gmask) cond += "(base & " + hex64(op.gmask) + ") == " + hex64(op.gval);
+        if (op.tmask) cond += std::string(cond.empty() ? "" : " && ") + "(t & " + hex32(op.tmask) + ") == " + hex32(op.tval);
+        if (cond.empty()) o += body;
+        else o += "        if (" + cond + ") {\n" + body + "        }\n";
+    }
+    for (int k = 0; k < A; ++k)
+        o += "        s[(base | " + hex64(pm.sreg[k]) + ") + st] = " + reg(k) + ";\n";
+    o += "    }\n}\n";
+    return o;
+}
+
+struct Compiled {
+    std::vector<char> binary;
+    double seconds;
+};
+
+std::mutex g_cache_mu;
+std::unordered_map<uint64_t, std::shared_ptr<Compiled>> g_cache;
+
+}  // namespace
+
+void jit_generate(const Plan &plan, JitCode *out) {
+    Gen g;
+    g.f32 = plan.dtype == HGM_C64;
+    g.V = g.f32 ? "hgm_f2" : "hgm_d2";
+    g.T = g.f32 ? "float" : "double";
+    out->source = "// generated pass kernels\n"
+                  "struct __attribute__((aligned(16))) hgm_d2 { double x, y; };\n"
+                  "struct __attribute__((aligned(8))) hgm_f2 { float x, y; };\n";
+    out->pass_kernel.clear();
+    out->names.clear();
+    std::map<std::string, int> seen;   // body (with a placeholder name) -> kernel index
+    for (const Pass &ps : plan.passes) {
+        if (ps.op_count == 0) { out->pass_kernel.push_back(-1); continue; }
+        std::string body = kernel_body(g, plan, ps, "@NAME@");
+        auto it = seen.find(body);
+        if (it != seen.end()) { out->pass_kernel.push_back(it->second); continue; }
+        const int idx = (int)out->names.size();
+        const std::string name = "hgm_pass_" + std::to_string(idx);
+        seen[body] = idx;
+        out->names.push_back(name);
+        out->pass_kernel.push_back(idx);
+        body.replace(body.find("@NAME@"), 6, name);
+        out->source += body;
+    }
+    out->hash = fnv1a(out->source);
+}
+
+int jit_compile(JitCode *out, std::string *err) {
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_cache.find(out->hash);
+        if (it != g_cache.end()) {
+            out->binary = it->second->binary;
+            out->compile_s = 0;
+            return HGM_OK;
+        }
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, out->source.c_str(), "hgm_passes.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        *err = "hiprtcCreateProgram failed";
+        return HGM_E_HIP;
+    }
+    const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast",
+                          // keep the compiler from emitting v_pk_{add,mul,fma}_f32
+                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    if (rc != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(prog, &log[0]);
+        *err = "hipRTC compilation of the pass kernels failed: " + log.substr(0, 2000);
+        hiprtcDestroyProgram(&prog);
+        return HGM_E_HIP;
+    }
+    size_t n = 0;
+    hiprtcGetCodeSize(prog, &n);
+    out->binary.assign(n, 0);
+    hiprtcGetCode(prog, out->binary.data());
+    hiprtcDestroyProgram(&prog);
+    out->compile_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    auto c = std::make_shared<Compiled>();
+    c->binary = out->binary;
+    c->seconds = out->compile_s;
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_cache[out->hash] = c;
+    return HGM_OK;
+}
+
+}  // namespace hgm

@@ -1,0 +1,35 @@
+// Plan-time specialisation of the register kernel.
+//
+// The register program of every pass (RegOp list, phase maps, load/store
+// assignments) is translated into straight-line HIP source: amplitudes in
+// named registers, gate matrices as exact hexadecimal literals (zero, +-1 and
+// real entries fold away), register- and global-bit conditions resolved at
+// generation time. The source is compiled for gfx950 with hipRTC when the plan
+// is built (no GPU needed) and loaded into a module on first use.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "planner.hpp"
+
+namespace hgm {
+
+struct JitCode {
+    std::string source;          // generated HIP source of all pass kernels
+    std::vector<char> binary;    // gfx950 code object
+    std::vector<int32_t> pass_kernel;   // kernel index of each pass (passes with identical code share one)
+    std::vector<std::string> names;     // kernel symbol names
+    uint64_t hash = 0;           // of the source
+    double compile_s = 0;
+};
+
+// Generates the source of every pass kernel of a register-kernel plan.
+void jit_generate(const Plan &plan, JitCode *out);
+
+// Compiles out->source with hipRTC for gfx950 (cached by source hash within
+// the process). Returns an HGM_* status.
+int jit_compile(JitCode *out, std::string *err);
+
+}  // namespace hgm

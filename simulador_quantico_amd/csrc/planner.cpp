@@ -4,6 +4,8 @@
 #include <chrono>
 #include <cstring>
 
+#include "jit.hpp"
+
 namespace hgm {
 
 namespace {
@@ -112,7 +114,7 @@ int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::
     if (!is_pow2(B) || !is_pow2(R)) { *err = "Erro de tuning: block_size and repeat_count must be powers of two"; return HGM_E_TUNING; }
     if (t.ops_block < 1) { *err = "Erro de tuning: ops_block must be >= 1"; return HGM_E_TUNING; }
     if (t.planner != HGM_PLAN_GREEDY && t.planner != HGM_PLAN_WINDOW) { *err = "unknown planner"; return HGM_E_TUNING; }
-    if (t.kernel < HGM_KERNEL_AUTO || t.kernel > HGM_KERNEL_REG) { *err = "unknown kernel family"; return HGM_E_TUNING; }
+    if (t.kernel < HGM_KERNEL_AUTO || t.kernel > HGM_KERNEL_JIT) { *err = "unknown kernel family"; return HGM_E_TUNING; }
     if ((size_t(1) << r) * amp_bytes(dtype) > kMaxLdsBytes - kLdsReserve) {
         *err = "Erro de tuning: a tile of 2^" + std::to_string(r) + " amplitudes exceeds the 160 KiB LDS of a gfx950 workgroup";
         return HGM_E_TUNING;
@@ -257,15 +259,20 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
     }
     const int A = 2 * plan.shape.R;
     const bool reg_ok = A <= kMaxRegAmps && plan.shape.B <= reg_kernel_max_threads(A);
-    if (t.kernel == HGM_KERNEL_REG && !reg_ok) {
+    if ((t.kernel == HGM_KERNEL_REG || t.kernel == HGM_KERNEL_JIT) && !reg_ok) {
         *err = "Erro de tuning: the register kernel holds at most " + std::to_string(kMaxRegAmps) +
                " amplitudes per thread and at most " + std::to_string(reg_kernel_max_threads(A)) +
                " threads for 2*repeat_count = " + std::to_string(A);
         return HGM_E_TUNING;
     }
-    if (t.kernel == HGM_KERNEL_REG || (t.kernel == HGM_KERNEL_AUTO && reg_ok)) {
-        plan.kernel = HGM_KERNEL_REG;
+    if (t.kernel == HGM_KERNEL_REG || t.kernel == HGM_KERNEL_JIT || (t.kernel == HGM_KERNEL_AUTO && reg_ok)) {
+        plan.kernel = t.kernel == HGM_KERNEL_JIT ? HGM_KERNEL_JIT : HGM_KERNEL_REG;
         build_register_program(plan);
+        if (plan.kernel == HGM_KERNEL_JIT) {
+            plan.jit = std::make_shared<JitCode>();
+            jit_generate(plan, plan.jit.get());
+            if (int rc = jit_compile(plan.jit.get(), err)) return rc;
+        }
     }
     plan.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = std::move(plan);

@@ -15,12 +15,15 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "hgm_gpu.h"
 
 namespace hgm {
+
+struct JitCode;
 
 enum OpKind : int32_t {
     K_DENSE = 0,        // full 2x2 update of the pair
@@ -135,6 +138,7 @@ struct Plan {
     std::vector<PhaseMap> maps;
     std::vector<PassMem> mems;
     int max_phases = 0;
+    std::shared_ptr<JitCode> jit;     // HGM_KERNEL_JIT: generated and compiled pass kernels
 };
 
 // Builds the register program of every pass (fills rops/maps/mems).

@@ -15,12 +15,14 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "hgm_gpu.h"
 #include "kernels.hpp"
 #include "planner.hpp"
 #include "runtime.hpp"
+#include "jit.hpp"
 
 struct hgm_state {
     void *d = nullptr;
@@ -63,12 +65,22 @@ struct Ctx {
     uint64_t *h_pidx = nullptr;
 };
 
+struct JitModule {
+    hipModule_t mod = nullptr;
+    std::vector<hipFunction_t> fns;
+};
+
 Ctx &ctx() {
     static Ctx c;
     return c;
 }
 
 constexpr int kRedBlocks = 1024;
+
+std::unordered_map<uint64_t, JitModule> &jit_modules() {
+    static std::unordered_map<uint64_t, JitModule> m;
+    return m;
+}
 
 int hip_fail(hipError_t e, const char *what) {
     t_err = std::string(what) + ": " + hipGetErrorString(e);
@@ -114,6 +126,24 @@ int init_locked(int device) {
     return HGM_OK;
 }
 
+// Loads a compiled pass module once per process (by source hash). Assumes the lock.
+int jit_load_locked(const JitCode &jc, JitModule **out) {
+    auto &mods = jit_modules();
+    auto it = mods.find(jc.hash);
+    if (it == mods.end()) {
+        JitModule m;
+        HGM_HIP(hipModuleLoadData(&m.mod, jc.binary.data()), "hipModuleLoadData(pass kernels)");
+        for (const std::string &name : jc.names) {
+            hipFunction_t f;
+            HGM_HIP(hipModuleGetFunction(&f, m.mod, name.c_str()), "hipModuleGetFunction");
+            m.fns.push_back(f);
+        }
+        it = mods.emplace(jc.hash, std::move(m)).first;
+    }
+    *out = &it->second;
+    return HGM_OK;
+}
+
 int reserve_ops(size_t n) {
     Ctx &c = ctx();
     if (n <= c.ops_cap) return HGM_OK;
@@ -148,8 +178,12 @@ int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st) {
         return HGM_E_STATE;
     }
     const bool reg = p.kernel == HGM_KERNEL_REG;
+    const bool jit = p.kernel == HGM_KERNEL_JIT;
     int rc;
-    if (reg) {
+    JitModule *jm = nullptr;
+    if (jit) {
+        if ((rc = jit_load_locked(*p.jit, &jm))) return rc;
+    } else if (reg) {
         if ((rc = reserve(&c.rops, &c.rops_cap, p.rops.size() * sizeof(RegOp), "hipMalloc(programs)"))) return rc;
         if ((rc = reserve(&c.maps, &c.maps_cap, p.maps.size() * sizeof(PhaseMap), "hipMalloc(maps)"))) return rc;
         if ((rc = reserve(&c.mems, &c.mems_cap, p.mems.size() * sizeof(PassMem), "hipMalloc(pass tables)"))) return rc;
@@ -174,11 +208,23 @@ int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st) {
     a.n_tiles = int64_t(1) << (p.qubits - p.shape.r);
     const uint64_t all = (uint64_t(1) << p.qubits) - 1;
     const size_t tile_bytes = (size_t(1) << p.shape.r) * amp_bytes(p.dtype);
+    const unsigned grid = (unsigned)(a.n_tiles < (int64_t(1) << 22) ? a.n_tiles : (int64_t(1) << 22));
     HGM_HIP(hipEventRecord(c.e0, c.stream), "hipEventRecord");
-    for (const Pass &ps : p.passes) {
+    for (size_t pi = 0; pi < p.passes.size(); ++pi) {
+        const Pass &ps = p.passes[pi];
         if (ps.op_count == 0) continue;
         a.nonlocal_mask = all & ~ps.local_mask;
-        if (reg) {
+        if (jit) {
+            // one tile per workgroup; LDS slots padded by 1/16 + 1/256
+            const uint64_t last = (uint64_t(1) << p.shape.r) - 1;
+            const size_t jit_lds = (size_t)(last + (last >> 4) + (last >> 8) + 1) * amp_bytes(p.dtype);
+            void *state = s->d;
+            void *args[] = {&state};
+            HGM_HIP(hipModuleLaunchKernel(jm->fns[p.jit->pass_kernel[pi]], (unsigned)a.n_tiles, 1, 1,
+                                          (unsigned)p.shape.B, 1, 1, ps.phases > 1 ? (unsigned)jit_lds : 0u,
+                                          c.stream, args, nullptr),
+                    "pass launch");
+        } else if (reg) {
             HGM_HIP(launch_pass_reg(p.dtype, 2 * p.shape.R, s->d, static_cast<const RegOp *>(c.rops) + ps.rop_begin,
                                     (int)ps.rop_count, static_cast<const PhaseMap *>(c.maps),
                                     static_cast<const PassMem *>(c.mems) + ps.mem, a.nonlocal_mask, a.n_tiles,
@@ -496,6 +542,16 @@ int hgm_plan_shape(const hgm_plan *p, int32_t *region_bits, int32_t *block_size,
 }
 
 int hgm_plan_kernel(const hgm_plan *p) { return p ? p->plan.kernel : -1; }
+
+int hgm_plan_jit_source(const hgm_plan *p, char *out, size_t cap, size_t *len) {
+    if (!p || !len) { t_err = "bad arguments"; return HGM_E_ARG; }
+    const std::string src = p->plan.jit ? p->plan.jit->source : std::string();
+    *len = src.size();
+    if (out && cap > src.size()) std::memcpy(out, src.c_str(), src.size() + 1);
+    return HGM_OK;
+}
+
+double hgm_plan_compile_seconds(const hgm_plan *p) { return (p && p->plan.jit) ? p->plan.jit->compile_s : 0.0; }
 
 int hgm_plan_program(const hgm_plan *p, int64_t i, void *rops, int64_t *n_rops, int32_t *first_map,
                      int32_t *last_map, int8_t *local_pos) {

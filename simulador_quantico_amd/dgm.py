@@ -303,6 +303,17 @@ class Plan:
     def kernel(self) -> int:
         return int(lib.hgm_plan_kernel(self._h))
 
+    @property
+    def compile_seconds(self) -> float:
+        return float(lib.hgm_plan_compile_seconds(self._h))
+
+    def jit_source(self) -> str:
+        n = C.c_size_t()
+        check(lib.hgm_plan_jit_source(self._h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(lib.hgm_plan_jit_source(self._h, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
+
     def program(self, i: int):
         """Register-kernel program of pass i: (structured ops, first map, last map, local_pos)."""
         n = C.c_int64()

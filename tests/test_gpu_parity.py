@@ -286,3 +286,43 @@ def test_gpu_execution_wrapper_reference_abi():
     want = oracle_run_gates(n, list(t) * 2, list(cm) * 2, list(cv) * 2,
                             [m.astype(np.complex128) for m in mats32] * 2, basis=5)
     assert np.abs(state - want).max() < TOL32
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
+def test_jit_kernel_parity(dtype, tol):
+    """Plan-time compiled pass kernels against the oracle (controls on register,
+    thread and tile bits, diagonal and anti-diagonal gates, several remaps)."""
+    n = 14
+    rng = np.random.default_rng(4242)
+    t, cm, cv, mats = _random_gate_list(rng, n, 160)
+    ops = hq.make_ops(t, cm, cv, mats)
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+    want = oracle_run_gates(n, t, cm, cv, mats, state=psi0)
+    for B, R, r, c in [(64, 8, 10, 3), (32, 4, 8, 2)]:
+        tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c,
+                               kernel=N.HGM_KERNEL_JIT, ops_block=60)
+        plan = hq.Plan(n, ops, tun, 1, dtype)
+        assert plan.kernel == N.HGM_KERNEL_JIT
+        s = hq.State(n, dtype)
+        s.upload(psi0)
+        plan.run(s)
+        say("jit", (B, R, r, c), dtype, "passes", len(plan))
+        assert np.abs(s.download() - want).max() < tol
+
+
+def test_jit_random_layers_30q_round_trip():
+    n = 30
+    d = hq.DGM()
+    d.setFunction(hq.RandomLayers(n, 20, 1, d))
+    ops = d.ops()
+    tun = N.default_tuning(block_size=128, repeat_count=8, gpu_region_bits=11, gpu_coalesced_bits=3,
+                           kernel=N.HGM_KERNEL_JIT)
+    s = hq.State(n)
+    s.set_basis(0)
+    st = hq.Plan(n, ops, tun).run(s)
+    say("jit random30 passes", st["passes"], "kernel_ms", st["kernel_ms"])
+    assert abs(s.norm2() - 1) < 1e-11
+    hq.Plan(n, inverse_ops(ops), tun).run(s)
+    idx, p = s.argmax()
+    assert idx == 0 and abs(p - 1) < 1e-11

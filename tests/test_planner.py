@@ -127,3 +127,18 @@ def test_grover_ops_plan_quickly():
     # three groups of 8 need two passes per n-controlled Z
     tun = N.default_tuning(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4)
     assert len(hq.Plan(n, ops, tun)) <= 4 * 50 + 3
+
+
+def test_jit_source_compiles_and_dedupes():
+    """The plan-time generated pass kernels compile for gfx950 (hipRTC, no GPU
+    needed); repeated iterations with identical passes share one kernel."""
+    n = 16
+    d = hq.DGM()
+    d.setFunction(hq.GroverIteration(n, 12345))
+    tun = N.default_tuning(block_size=64, repeat_count=8, gpu_region_bits=10, gpu_coalesced_bits=3,
+                           kernel=N.HGM_KERNEL_JIT)
+    plan = hq.Plan(n, d.ops(), tun, iterations=20)
+    src = plan.jit_source()
+    kernels = src.count("__global__")
+    assert plan.kernel == N.HGM_KERNEL_JIT and 0 < kernels < len(plan)
+    assert "v_pk" not in src and "hgm_pass_0" in src
