@@ -17,6 +17,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -45,6 +46,9 @@ WORKLOADS = {
     "draper29": dict(qubits=29, dtype="c128", kind="draper", seed=29,
                      tuning=dict(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=4)),
 }
+
+
+KERNEL_NAMES = {1: "k_pass_lds", 2: "k_pass_reg", 3: "hgm_pass_* (plan-compiled)"}
 
 
 def build_circuit(w):
@@ -122,6 +126,71 @@ def cpu_baseline(max_seconds: float = 20.0):
                       f"({gates} gates) at {n} qubits, {dt:.1f} s, {threads} threads"}
 
 
+def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026, autotune: bool = False,
+                  fidelity_sample: int = 24) -> dict:
+    """The serving mix on the batched server (FP32 states) against sequential
+    plain FP64 one-circuit-per-call runs of the same request stream."""
+    from simulador_quantico_amd import _native as N
+    from simulador_quantico_amd import serving as S
+    t0 = time.perf_counter()
+    jobs = S.make_mix(requests, rate=rate, seed=seed)
+    warm = S.make_mix(64, rate=rate, seed=seed + 1, qmax=20)
+    build_s = time.perf_counter() - t0
+    print(f"bench: serving mix of {requests} requests built in {build_s:.1f} s", file=sys.stderr, flush=True)
+    with S.Server(np.complex64, max_active=256, hbm_budget_gib=64, autotune=autotune) as srv:
+        S.run_served(warm, srv, time_scale=0.0)
+        res = S.run_served(jobs, srv)
+        # fidelity of FP32 results against FP64 on a sample of the same stream (untimed)
+        sample = [j for j in jobs if j.qubits <= 20][:fidelity_sample]
+        outs = {j.index: np.zeros(1 << j.qubits, np.complex64) for j in sample}
+        for j in sample:
+            srv.submit(j.qubits, j.ops, j.basis, state_out=outs[j.index])
+        srv.wait_all()
+        srv.poll(cap=len(sample) + 8)
+    bad = [r for r in res if r["status"] != 0]
+    lat = np.array([r["latency_ms"] for r in res])
+    t_first = min(r["t_submit_ms"] for r in res)
+    t_last = max(r["t_done_ms"] for r in res)
+    served_cps = len(res) / ((t_last - t_first) / 1e3)
+    by_job = {r["job"]: r for r in res}
+    adders = [j for j in jobs if j.kind == "adder"]
+    adder_ok = sum(by_job[j.index]["argmax"] == j.expect for j in adders)
+    print(f"bench: served {len(res)} in {(t_last - t_first) / 1e3:.2f} s", file=sys.stderr, flush=True)
+
+    S.run_sequential(warm[:4])
+    seq_jobs = jobs[:seq_sample]
+    seq = S.run_sequential(seq_jobs)
+    service = np.array([r["service_ms"] for r in seq])
+    seq_cps = len(seq) / (service.sum() / 1e3)
+    seq_lat = S.queue_latency([j.arrival_s for j in seq_jobs], service)
+    agree = sum(by_job[r["job"]]["argmax"] == r["argmax"] for r in seq)
+    print(f"bench: sequential {len(seq)} in {service.sum() / 1e3:.2f} s", file=sys.stderr, flush=True)
+
+    fids = []
+    for j in sample:
+        ref = np.zeros(1 << j.qubits, np.complex128)
+        ref[j.basis] = 1
+        tun = S.sequential_tuning(j.qubits)
+        st = N.Stats()
+        N.check(N.lib.hgm_execute(ref.ctypes.data, j.qubits, N.HGM_C128, S._op_ptr(j.ops), len(j.ops),
+                                  C.byref(tun), 1, C.byref(st)))
+        fids.append(S.fidelity(ref, outs[j.index]))
+    return {
+        "requests": len(res), "failed": len(bad), "arrival_rate_per_s": rate, "dtype": "f32",
+        "circuits_per_s": served_cps, "p50_latency_ms": float(np.percentile(lat, 50)),
+        "p99_latency_ms": float(np.percentile(lat, 99)), "makespan_s": (t_last - t_first) / 1e3,
+        "sequential_fp64": {"sample": len(seq), "circuits_per_s": seq_cps,
+                            "p50_latency_ms": float(np.percentile(seq_lat, 50)),
+                            "p99_latency_ms": float(np.percentile(seq_lat, 99)),
+                            "mean_service_ms": float(service.mean()), "argmax_agree": agree},
+        "speedup_circuits_per_s": served_cps / seq_cps,
+        "adder_sums_exact": f"{adder_ok}/{len(adders)}",
+        "fidelity_fp32_vs_fp64_min": float(min(fids)) if fids else None, "fidelity_sample": len(fids),
+        "mix": "Poisson arrivals; qubits uniform in [10, 26]; uniform over Hadamard layer, Grover (1-64 "
+               "iterations), Draper adder, random layers depth 10; seed %d" % seed,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,7 +201,13 @@ def main():
     ap.add_argument("--planner", type=int, default=None)
     ap.add_argument("--ops-block", type=int, default=None)
     ap.add_argument("--tuning", default=None, help="B,R,r,c override")
-    ap.add_argument("--kernel", type=int, default=None, help="0 auto, 1 LDS, 2 register")
+    ap.add_argument("--kernel", type=int, default=None, help="0 auto, 1 LDS, 2 register, 3 plan-compiled")
+    ap.add_argument("--serving", action="store_true", help="run only the serving mix and print its JSON line")
+    ap.add_argument("--requests", type=int, default=4096)
+    ap.add_argument("--rate", type=float, default=2000.0, help="serving mix arrival rate (requests/s)")
+    ap.add_argument("--seq-sample", type=int, default=256,
+                    help="requests of the same stream run by the sequential FP64 baseline")
+    ap.add_argument("--autotune", action="store_true", help="serving: autotune tile shapes per qubit count")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain device copy of the workload's state (the known-size reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters) instead of the circuit")
@@ -144,6 +219,13 @@ def main():
 
     import simulador_quantico_amd as hq
     from simulador_quantico_amd import _native as N
+
+    if args.serving:
+        out = serving_bench(args.requests, args.rate, args.seq_sample, autotune=args.autotune)
+        print(json.dumps({"metric": METRIC, "value": out["circuits_per_s"], "unit": "circuits/s", "n_gpus": 1,
+                          "higher_is_better": True, "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": "serving mix"}, "serving": out}))
+        return
 
     if args.copy_calibration:
         w = WORKLOADS[args.workload]
@@ -216,10 +298,10 @@ def main():
         "data": "synthetic (seeded circuit, state |0..0> resident in HBM)",
         "config": {"workload": name, "qubits": n, "gates_per_step": gates, "passes_per_step": len(plan),
                    "tile": {k: v for k, v in tun.items()}, "planner": "greedy" if tun.get("planner", 0) == 0
-                   else "window", "state_norm_after": norm},
+                   else "window", "plan_compile_s": plan.compile_seconds, "state_norm_after": norm},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_pass_reg" if plan.kernel == N.HGM_KERNEL_REG else "k_pass_lds",
+                     "kernel": KERNEL_NAMES.get(plan.kernel, "?"),
                      "bytes_per_launch": bytes_per_launch,
                      "mean_launch_ms": mean_launch_ms},
     }

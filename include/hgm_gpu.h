@@ -196,6 +196,76 @@ double hgm_plan_compile_seconds(const hgm_plan *p);
  * HGM_E_TUNING with a message when the tuple is rejected. */
 int hgm_validate_tuning(int qubits, int dtype, const hgm_tuning *t);
 
+/* ------------------------------------------------------------------ autotuner
+ * Times every valid tile shape (2*repeat_count in {8, 16, 32} amplitudes per
+ * thread, gpu_region_bits 8..13, gpu_coalesced_bits 2..4) of the given kernel
+ * family (AUTO, REG or LDS) on a scratch device state running the given
+ * circuit, and returns the fastest tuning and its kernel time. No counterpart
+ * in the reference, whose tuning is given by hand on the command line
+ * (include/cli/cli_common.h:21-25). */
+int hgm_autotune(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, int kernel,
+                 hgm_tuning *best, double *best_ms);
+
+/* ------------------------------------------------------------------ serving
+ * Continuous batching of many circuit requests on one GPU. No counterpart in
+ * the reference (one DGM executes one circuit per execute() call). Requests
+ * are planned on host threads as they arrive, admitted in arrival order while
+ * their HBM footprint fits the budget and a slot is free, and then advance
+ * together: each step runs the next pass of every admitted state in batched
+ * launches. A finished request reports its readout (FP64 norm and the argmax
+ * of |a|^2) and, when asked, its final state. Each result is computed by the
+ * same kernels and programs as when the request runs alone, so it does not
+ * depend on what else is in flight. Thread-safe: any thread may submit and
+ * poll. */
+typedef struct hgm_server hgm_server;
+
+typedef struct hgm_server_config {
+    int32_t dtype;               /* amplitude type of every request (default HGM_C64) */
+    int32_t max_active;          /* requests resident on the device at once (default 256) */
+    uint64_t hbm_budget_bytes;   /* device memory for states, programs and readout (default 32 GiB) */
+    int32_t planner_threads;     /* host threads that plan requests (default 4) */
+    int32_t autotune;            /* 1: time tile shapes per qubit count at creation (hgm_autotune) */
+    int32_t min_qubits;          /* qubit range to autotune (default 10..26) */
+    int32_t max_qubits;
+} hgm_server_config;
+
+typedef struct hgm_request {
+    int32_t qubits;
+    int32_t iterations;          /* >= 1: the gate list is applied this many times */
+    const hgm_op *ops;           /* copied at submission */
+    int64_t n_ops;               /* >= 1 */
+    uint64_t basis;              /* initial state |basis> */
+    void *state_out;             /* optional host buffer for the final state (2^qubits amplitudes) */
+} hgm_request;
+
+typedef struct hgm_result {
+    int64_t id;
+    int32_t status;              /* HGM_OK or the planning / launch error */
+    int32_t passes;              /* passes over the state */
+    uint64_t argmax;             /* index of the largest |a|^2 (lowest index on ties) */
+    double p_max;                /* that |a|^2 */
+    double norm2;                /* sum of |a|^2 (FP64) */
+    double t_submit_ms;          /* server clock: submission, admission, result ready on the host */
+    double t_admit_ms;
+    double t_done_ms;
+} hgm_result;
+
+void hgm_server_default_config(hgm_server_config *c);
+int hgm_server_create(const hgm_server_config *c, hgm_server **out);
+/* Queue a request; *id numbers requests in submission order from 0. */
+int hgm_server_submit(hgm_server *s, const hgm_request *r, int64_t *id);
+/* Up to cap finished results in completion order; waits up to timeout_ms for
+ * the first one (0: return at once). */
+int hgm_server_poll(hgm_server *s, hgm_result *out, int cap, int *n, int timeout_ms);
+/* Block until every submitted request has finished (results stay pollable). */
+int hgm_server_wait_all(hgm_server *s);
+/* Tuning the server uses for states of `qubits` qubits. */
+int hgm_server_tuning(const hgm_server *s, int qubits, hgm_tuning *out);
+/* Milliseconds since the server was created (the clock of hgm_result). */
+double hgm_server_clock_ms(const hgm_server *s);
+/* Finishes every submitted request, then frees the server. */
+void hgm_server_destroy(hgm_server *s);
+
 /* ------------------------------------------------------------------ DGM handle
  * The executor API of the reference (class DGM, include/core/dgm.h:65-163)
  * behind an opaque handle, for callers without C++ (ctypes, cgo). Fields are

@@ -52,7 +52,28 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("max_active", C.c_int32), ("hbm_budget_bytes", C.c_uint64),
+                ("planner_threads", C.c_int32), ("autotune", C.c_int32), ("min_qubits", C.c_int32),
+                ("max_qubits", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("qubits", C.c_int32), ("iterations", C.c_int32), ("ops", C.POINTER(Op)), ("n_ops", C.c_int64),
+                ("basis", C.c_uint64), ("state_out", C.c_void_p)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("passes", C.c_int32), ("argmax", C.c_uint64),
+                ("p_max", C.c_double), ("norm2", C.c_double), ("t_submit_ms", C.c_double),
+                ("t_admit_ms", C.c_double), ("t_done_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 assert C.sizeof(Op) == 88 and C.sizeof(Tuning) == 32
+assert C.sizeof(ServerConfig) == 32 and C.sizeof(Request) == 40 and C.sizeof(Result) == 64
 
 # (name, restype, argtypes) of every exported entry point of include/hgm_gpu.h
 _P = C.c_void_p
@@ -120,6 +141,16 @@ SIGNATURES = [
     ("hgm_seed_rng", None, [C.c_uint64]),
     ("hgm_generate", C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64), C.c_int, C.c_char_p, C.c_char_p, C.c_size_t,
                                C.POINTER(C.c_size_t)]),
+    ("hgm_autotune", C.c_int, [C.c_int, C.c_int, C.POINTER(Op), C.c_int64, C.c_int, C.c_int, C.POINTER(Tuning),
+                               C.POINTER(C.c_double)]),
+    ("hgm_server_default_config", None, [C.POINTER(ServerConfig)]),
+    ("hgm_server_create", C.c_int, [C.POINTER(ServerConfig), C.POINTER(_P)]),
+    ("hgm_server_submit", C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_int64)]),
+    ("hgm_server_poll", C.c_int, [_P, C.POINTER(Result), C.c_int, C.POINTER(C.c_int), C.c_int]),
+    ("hgm_server_wait_all", C.c_int, [_P]),
+    ("hgm_server_tuning", C.c_int, [_P, C.c_int, C.POINTER(Tuning)]),
+    ("hgm_server_clock_ms", C.c_double, [_P]),
+    ("hgm_server_destroy", None, [_P]),
     ("setDevice", C.c_bool, [C.c_int]),
     ("GpuExecutionWrapper", _P, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     ("ProjectState", C.c_bool, [_P, C.c_int, C.c_int, C.c_long, C.c_long, C.c_int]),

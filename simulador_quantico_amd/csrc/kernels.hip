@@ -328,6 +328,126 @@ __global__ void __launch_bounds__(reg_kernel_max_threads(A)) k_pass_reg(typename
     }
 }
 
+// Batched register kernel: one launch runs one pass of each of many states
+// (of any qubit count that shares the workgroup shape). Workgroup b belongs to
+// the item whose tile range [tile_begin, next tile_begin) holds b; the items
+// are sorted by tile_begin, so a binary search on tile-uniform values finds it.
+// When the item asks for a readout, each tile also reduces its output (FP64
+// sum of |a|^2, largest |a|^2 with the lowest index on ties) into
+// partials[tile]; k_readout_final combines the tiles of a state.
+template <class T, int A>
+__global__ void __launch_bounds__(reg_kernel_max_threads(A)) k_batch_reg(const BatchItem *__restrict__ items,
+                                                                          int n_items) {
+    using V = typename Vec<T>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double red_s[16], red_m[16];
+    __shared__ uint64_t red_i[16];
+    V *lds = reinterpret_cast<V *>(smem);
+    const int t = threadIdx.x;
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = n_items - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].tile_begin <= b) lo = mid; else hi = mid - 1;
+    }
+    const BatchItem &it = items[lo];
+    V *__restrict__ state = reinterpret_cast<V *>(it.state);
+    const RegOp *__restrict__ ops = it.ops;
+    const PhaseMap *__restrict__ maps = it.maps;
+    const PassMem *__restrict__ pm = it.pm;
+    const int n_ops = it.n_ops;
+    const int64_t tid = b - it.tile_begin;
+    const uint64_t base = pdep64((uint64_t)tid, it.nonlocal_mask);
+    const uint64_t lt = pm->lthr[0][t & 31] | pm->lthr[1][(t >> 5) & 31];
+    const uint64_t st = pm->sthr[0][t & 31] | pm->sthr[1][(t >> 5) & 31];
+    V a[A];
+#pragma unroll
+    for (int k = 0; k < A; ++k) a[k] = state[base | lt | pm->lreg[k]];
+    int cur = pm->first_map;
+    for (int i = 0; i < n_ops; ++i) {
+        const RegOp &o = ops[i];
+        if (o.kind == R_REMAP) {
+            reg_remap<T, A>(a, lds, maps[cur], maps[o.rbit], t);
+            cur = o.rbit;
+            continue;
+        }
+        if ((base & o.gmask) != o.gval) continue;
+        if (o.kind == R_PHASE || o.kind == R_PHASE_REAL) reg_phase<T, A>(a, o, t);
+        else reg_gate<T, A>(a, o, t);
+    }
+#pragma unroll
+    for (int k = 0; k < A; ++k) state[base | st | pm->sreg[k]] = a[k];
+    if (it.partials == nullptr) return;
+    double sum = 0.0, best = -1.0;
+    uint64_t bi = ~uint64_t(0);
+#pragma unroll
+    for (int k = 0; k < A; ++k) {
+        const double p = (double)a[k].x * (double)a[k].x + (double)a[k].y * (double)a[k].y;
+        const uint64_t idx = base | st | pm->sreg[k];
+        sum += p;
+        if (p > best || (p == best && idx < bi)) { best = p; bi = idx; }
+    }
+    // lanes of a wave first (only lanes that exist: width = min(64, blockDim)), then waves in order
+    const int width = blockDim.x < 64 ? (int)blockDim.x : 64;
+    for (int off = width >> 1; off > 0; off >>= 1) {
+        sum += __shfl_xor(sum, off, width);
+        const double ob = __shfl_xor(best, off, width);
+        const uint64_t oi = __shfl_xor(bi, off, width);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    if ((t & 63) == 0) { red_s[t >> 6] = sum; red_m[t >> 6] = best; red_i[t >> 6] = bi; }
+    __syncthreads();
+    if (t == 0) {
+        const int waves = (blockDim.x + 63) >> 6;
+        for (int w = 1; w < waves; ++w) {
+            sum += red_s[w];
+            if (red_m[w] > best || (red_m[w] == best && red_i[w] < bi)) { best = red_m[w]; bi = red_i[w]; }
+        }
+        ReadoutPartial r;
+        r.sum = sum;
+        r.pmax = best;
+        r.idx = bi;
+        r.pad = 0;
+        it.partials[tid] = r;
+    }
+}
+
+// One workgroup per task: combines a state's per-tile readout partials in
+// tile order.
+__global__ void __launch_bounds__(256) k_readout_final(const ReadoutTask *__restrict__ tasks) {
+    __shared__ double ss[256], sm[256];
+    __shared__ uint64_t si[256];
+    const ReadoutTask &tk = tasks[blockIdx.x];
+    double sum = 0.0, best = -1.0;
+    uint64_t bi = ~uint64_t(0);
+    for (int64_t i = threadIdx.x; i < tk.n; i += 256) {
+        const ReadoutPartial p = tk.partials[i];
+        sum += p.sum;
+        if (p.pmax > best || (p.pmax == best && p.idx < bi)) { best = p.pmax; bi = p.idx; }
+    }
+    ss[threadIdx.x] = sum;
+    sm[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ss[threadIdx.x] += ss[threadIdx.x + w];
+            const double o = sm[threadIdx.x + w];
+            const uint64_t oi = si[threadIdx.x + w];
+            if (o > sm[threadIdx.x] || (o == sm[threadIdx.x] && oi < si[threadIdx.x])) { sm[threadIdx.x] = o; si[threadIdx.x] = oi; }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ReadoutPartial r;
+        r.sum = ss[0];
+        r.pmax = sm[0];
+        r.idx = si[0];
+        r.pad = 0;
+        *tk.out = r;
+    }
+}
+
 // Plain streaming copy, 16 bytes per lane per iteration.
 __global__ void __launch_bounds__(256) k_copy(uint4 *__restrict__ dst, const uint4 *__restrict__ src, uint64_t n16) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
@@ -457,6 +577,28 @@ hipError_t pass_reg_any(int A, void *state, const RegOp *ops, int n_ops, const P
     }
 }
 
+template <class T, int A>
+hipError_t batch_reg(const BatchItem *items, int n_items, int64_t total_tiles, int B, size_t lds, hipStream_t s) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipFuncSetAttribute((const void *)k_batch_reg<T, A>, hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_batch_reg<T, A>), dim3((unsigned)total_tiles), dim3(B), lds, s, items, n_items);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t batch_reg_any(int A, const BatchItem *items, int n_items, int64_t total_tiles, int B, size_t lds,
+                         hipStream_t s) {
+    switch (A) {
+        case 8: return batch_reg<T, 8>(items, n_items, total_tiles, B, lds, s);
+        case 16: return batch_reg<T, 16>(items, n_items, total_tiles, B, lds, s);
+        case 32: return batch_reg<T, 32>(items, n_items, total_tiles, B, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace
 
 hipError_t launch_pass_reg(int dtype, int A, void *state, const RegOp *ops, int n_ops, const PhaseMap *maps,
@@ -512,6 +654,20 @@ hipError_t launch_collapse(int dtype, void *state, uint64_t n, int bit, int outc
         hipLaunchKernelGGL(k_collapse<float>, dim3(grid), dim3(blk), 0, s, (float2 *)state, half, bit, outcome, norm);
     else
         hipLaunchKernelGGL(k_collapse<double>, dim3(grid), dim3(blk), 0, s, (double2 *)state, half, bit, outcome, norm);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reg(int dtype, int A, int B, const BatchItem *items, int n_items, int64_t total_tiles,
+                            size_t lds_bytes, hipStream_t s) {
+    if (n_items < 1 || total_tiles < 1 || total_tiles > 0x7fffffffLL || B < 1 || B > reg_kernel_max_threads(A))
+        return hipErrorInvalidValue;
+    return dtype == HGM_C64 ? batch_reg_any<float>(A, items, n_items, total_tiles, B, lds_bytes, s)
+                            : batch_reg_any<double>(A, items, n_items, total_tiles, B, lds_bytes, s);
+}
+
+hipError_t launch_readout_final(const ReadoutTask *tasks, int n_tasks, hipStream_t s) {
+    if (n_tasks < 1) return hipSuccess;
+    hipLaunchKernelGGL(k_readout_final, dim3((unsigned)n_tasks), dim3(256), 0, s, tasks);
     return hipGetLastError();
 }
 

@@ -26,6 +26,46 @@ hipError_t launch_pass_reg(int dtype, int A, void *state, const RegOp *ops, int 
                            const PassMem *pm, uint64_t nonlocal, int64_t n_tiles, int r, int B, size_t lds_bytes,
                            hipStream_t s);
 
+// Readout of one tile or of a whole state: FP64 sum of |a|^2 and the largest
+// |a|^2 with its index (lowest index on ties).
+struct ReadoutPartial {
+    double sum;
+    double pmax;
+    uint64_t idx;
+    uint64_t pad;
+};
+static_assert(sizeof(ReadoutPartial) == 32, "shared with the kernels");
+
+// One pass of one state inside a batched register-kernel launch.
+struct BatchItem {
+    void *state;
+    const RegOp *ops;            // the pass's register program
+    const PhaseMap *maps;        // the plan's phase maps
+    const PassMem *pm;           // the pass's load/store table
+    uint64_t nonlocal_mask;      // state-index bits that number the tiles
+    int64_t tile_begin;          // first workgroup of this item in the launch
+    ReadoutPartial *partials;    // non-null: per-tile readout of the pass output
+    int32_t n_ops;
+    int32_t pad;
+};
+static_assert(sizeof(BatchItem) == 64, "shared with the kernels");
+
+struct ReadoutTask {
+    const ReadoutPartial *partials;
+    int64_t n;
+    ReadoutPartial *out;
+};
+
+// One launch of the register kernel over several states (items sorted by
+// tile_begin; workgroups [0, total_tiles)); every item's plan has the same
+// A = 2*repeat_count and block size B. lds_bytes covers the largest tile of
+// an item with remaps.
+hipError_t launch_batch_reg(int dtype, int A, int B, const BatchItem *items, int n_items, int64_t total_tiles,
+                            size_t lds_bytes, hipStream_t s);
+
+// out of each task = combination of its n partials (one workgroup per task).
+hipError_t launch_readout_final(const ReadoutTask *tasks, int n_tasks, hipStream_t s);
+
 // dst[0..bytes) = src[0..bytes) (bytes a multiple of 16).
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 

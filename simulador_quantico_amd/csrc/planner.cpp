@@ -10,7 +10,6 @@ namespace hgm {
 
 namespace {
 
-constexpr int kMaxQubits = 40;
 constexpr size_t kMaxLdsBytes = 160 * 1024;   // gfx950 LDS per workgroup
 constexpr size_t kLdsReserve = 2 * 1024;      // bit-deposit tables of the LDS kernel
 

@@ -47,6 +47,7 @@ struct DevOp {
 };
 static_assert(sizeof(DevOp) == 96, "DevOp layout is shared with the kernels");
 
+constexpr int kMaxQubits = 40;
 constexpr int kMaxTileBits = 16;
 constexpr int kMaxRegAmps = 32;      // amplitudes per thread of the register kernel
 constexpr int kMaxThreadBits = 10;   // 1024 threads per workgroup

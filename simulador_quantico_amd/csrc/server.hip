@@ -1,0 +1,715 @@
+// Serving engine: continuous batching of circuit requests on one GPU, and the
+// tile-shape autotuner (the hgm_server_* / hgm_autotune part of hgm_gpu.h).
+//
+// Requests are planned on host threads as they arrive, admitted in arrival
+// order while their HBM footprint (state + register programs + readout
+// partials) fits the budget and a slot is free, and then advance together:
+// every step is one batched register-kernel launch per workgroup shape that
+// runs the next pass of every admitted state. States that fit one tile (up to
+// 13 qubits) run their whole circuit in one pass and live on a lane (stream)
+// of their own, so that long single-tile programs never hold back the passes
+// of large states. A state's last pass also reduces its readout (norm and
+// argmax, FP64); the step's results come back in one small copy.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "hgm_gpu.h"
+#include "kernels.hpp"
+#include "planner.hpp"
+#include "runtime.hpp"
+
+namespace hgm {
+namespace {
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
+
+std::string hip_msg(hipError_t e, const char *what) { return std::string(what) + ": " + hipGetErrorString(e); }
+
+// First-fit allocator over one device allocation; freed ranges merge with
+// their neighbours.
+class Arena {
+  public:
+    int init(size_t bytes, std::string *err) {
+        hipError_t e = hipMalloc(&base_, bytes);
+        if (e != hipSuccess) { *err = hip_msg(e, "hipMalloc(serving arena)"); base_ = nullptr; return HGM_E_HIP; }
+        size_ = bytes;
+        free_[0] = bytes;
+        return HGM_OK;
+    }
+    void destroy() {
+        if (base_) hipFree(base_);
+        base_ = nullptr;
+        free_.clear();
+    }
+    bool alloc(size_t bytes, size_t *off) {
+        bytes = align_up(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const size_t rest = it->second - bytes, at = it->first + bytes;
+            free_.erase(it);
+            if (rest) free_[at] = rest;
+            used_ += bytes;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = align_up(bytes);
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    char *base() const { return base_; }
+    size_t size() const { return size_; }
+    size_t used() const { return used_; }
+
+  private:
+    char *base_ = nullptr;
+    size_t size_ = 0, used_ = 0;
+    std::map<size_t, size_t> free_;
+};
+
+struct Req {
+    int64_t id = 0;
+    int qubits = 0;
+    int iterations = 1;
+    uint64_t basis = 0;
+    void *state_out = nullptr;
+    std::vector<hgm_op> ops;
+    double t_submit = 0, t_admit = -1, t_done = -1;
+    std::atomic<int> planned{0};   // 0 queued, 1 planned, 2 failed
+    int status = HGM_OK;
+    std::string err;
+    Plan plan;
+    std::vector<int> passes;       // non-empty passes in order
+    size_t next = 0;
+    int lane = 0;
+    int64_t n_tiles = 1;
+    size_t state_bytes = 0, prog_bytes = 0, part_bytes = 0, total_bytes = 0, off = 0;
+    char *d = nullptr;             // device block: state | programs | maps | pass tables | partials
+    ReadoutPartial result{};
+};
+
+struct Step {
+    std::vector<Req *> finishing;
+    int slot = 0;
+};
+
+struct Lane {
+    hipStream_t stream = nullptr;
+    std::vector<Req *> active;
+    std::deque<Step> inflight;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    char *d_ring = nullptr;        // 2 slots: items | tasks | results
+    char *h_ring = nullptr;        // pinned mirror
+    int next_slot = 0;
+};
+
+constexpr int kLanes = 2;          // 0: single-tile states (whole circuit per pass), 1: multi-tile states
+constexpr int kDepth = 2;          // steps in flight per lane
+
+double now_ms(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Tile shape the serving engine uses for a qubit count when no autotuning is
+// asked for: 16 amplitudes per thread; states up to 13 qubits are one tile;
+// larger ones use tiles of 2^12 amplitudes with 3 coalesced bits.
+hgm_tuning serving_default_tuning(int n) {
+    hgm_tuning t;
+    hgm_default_tuning(&t);
+    t.ops_block = 1 << 30;
+    t.kernel = HGM_KERNEL_REG;
+    if (n <= 13) {
+        const int64_t pairs = int64_t(1) << (n - 1);
+        t.repeat_count = (int)std::min<int64_t>(8, pairs);
+        t.block_size = (int)(pairs / t.repeat_count);
+        t.gpu_region_bits = n;
+        t.gpu_coalesced_bits = std::min(3, n);
+    } else {
+        t.block_size = 256;
+        t.repeat_count = 8;
+        t.gpu_region_bits = 12;
+        t.gpu_coalesced_bits = 3;
+    }
+    return t;
+}
+
+// A timing circuit for the autotuner: `layers` layers of a dense gate on every
+// qubit followed by a CZ brickwork (the random-layer structure).
+std::vector<hgm_op> timing_circuit(int n, int layers) {
+    std::vector<hgm_op> ops;
+    const double s = std::sqrt(0.5);
+    for (int l = 0; l < layers; ++l) {
+        for (int q = 0; q < n; ++q) {
+            hgm_op o{};
+            o.target_bit = n - 1 - q;
+            const double ph = 0.37 * (l + 1) + 0.11 * q;
+            const double c = std::cos(ph), sn = std::sin(ph);
+            const double m[8] = {s, 0, s * c, s * sn, s * c, -s * sn, -s, 0};
+            std::memcpy(o.m, m, sizeof m);
+            ops.push_back(o);
+        }
+        for (int q = l & 1; q + 1 < n; q += 2) {
+            hgm_op o{};
+            o.target_bit = n - 2 - q;
+            o.control_mask = o.control_value = uint64_t(1) << (n - 1 - q);
+            const double m[8] = {1, 0, 0, 0, 0, 0, -1, 0};
+            std::memcpy(o.m, m, sizeof m);
+            ops.push_back(o);
+        }
+    }
+    return ops;
+}
+
+}  // namespace
+}  // namespace hgm
+
+using namespace hgm;
+
+struct hgm_server {
+    hgm_server_config cfg{};
+    int device = 0;
+    std::chrono::steady_clock::time_point t0;
+    Arena arena;
+    Lane lanes[kLanes];
+    size_t slot_bytes = 0, tasks_off = 0, results_off = 0;
+    std::vector<hgm_tuning> tune;             // per qubit count
+
+    std::mutex mu;                            // submissions, results, plan queue
+    std::condition_variable cv_work, cv_done, cv_plan;
+    std::deque<Req *> submitted;              // not yet seen by the worker
+    std::deque<Req *> plan_q;
+    std::deque<hgm_result> done;
+    int64_t next_id = 0, n_submitted = 0, n_done = 0;
+    bool stop = false;
+    std::string fatal;                        // first device error
+
+    // worker-only state
+    std::deque<Req *> pending;                // arrival order, awaiting admission
+    int active = 0;
+
+    std::thread worker;
+    std::vector<std::thread> planners;
+};
+
+namespace {
+
+void finish(hgm_server *s, Req *r) {
+    hgm_result res{};
+    res.id = r->id;
+    res.status = r->status;
+    res.passes = (int32_t)r->passes.size();
+    res.argmax = r->result.idx;
+    res.p_max = r->result.pmax;
+    res.norm2 = r->result.sum;
+    res.t_submit_ms = r->t_submit;
+    res.t_admit_ms = r->t_admit;
+    res.t_done_ms = r->t_done;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->done.push_back(res);
+        ++s->n_done;
+    }
+    s->cv_done.notify_all();
+    delete r;
+}
+
+void fail(hgm_server *s, Req *r, int status, const std::string &msg) {
+    r->status = status;
+    r->err = msg;
+    r->t_done = now_ms(s->t0);
+    finish(s, r);
+}
+
+void planner_loop(hgm_server *s) {
+    for (;;) {
+        Req *r;
+        {
+            std::unique_lock<std::mutex> g(s->mu);
+            s->cv_plan.wait(g, [&] { return s->stop || !s->plan_q.empty(); });
+            if (s->plan_q.empty()) return;
+            r = s->plan_q.front();
+            s->plan_q.pop_front();
+        }
+        hgm_tuning t = s->tune[r->qubits];
+        std::string err;
+        int rc = build_plan(r->qubits, s->cfg.dtype, r->ops.data(), (int64_t)r->ops.size(), r->iterations, t,
+                            &r->plan, &err);
+        if (rc == HGM_OK && r->plan.kernel != HGM_KERNEL_REG) {
+            rc = HGM_E_TUNING;
+            err = "serving needs the register kernel for this tile shape";
+        }
+        if (rc == HGM_OK) {
+            for (size_t i = 0; i < r->plan.passes.size(); ++i)
+                if (r->plan.passes[i].op_count > 0) r->passes.push_back((int)i);
+            if (r->passes.empty()) { rc = HGM_E_ARG; err = "the request applies no gate"; }
+        }
+        if (rc == HGM_OK) {
+            const Plan &p = r->plan;
+            r->n_tiles = int64_t(1) << (r->qubits - p.shape.r);
+            r->lane = r->n_tiles == 1 ? 0 : 1;
+            r->state_bytes = align_up((size_t(1) << r->qubits) * amp_bytes(s->cfg.dtype));
+            r->prog_bytes = align_up(p.rops.size() * sizeof(RegOp)) + align_up(p.maps.size() * sizeof(PhaseMap)) +
+                            align_up(p.mems.size() * sizeof(PassMem));
+            r->part_bytes = align_up((size_t)r->n_tiles * sizeof(ReadoutPartial));
+            r->total_bytes = r->state_bytes + r->prog_bytes + r->part_bytes;
+            if (r->total_bytes > s->arena.size()) {
+                rc = HGM_E_ARG;
+                err = "the request's footprint exceeds the serving HBM budget";
+            }
+        }
+        r->status = rc;
+        r->err = err;
+        r->planned.store(rc == HGM_OK ? 1 : 2, std::memory_order_release);
+        s->cv_work.notify_all();
+    }
+}
+
+// Admit planned requests in arrival order while memory and slots allow.
+bool admit(hgm_server *s) {
+    bool any = false;
+    for (auto it = s->pending.begin(); it != s->pending.end();) {
+        Req *r = *it;
+        const int st = r->planned.load(std::memory_order_acquire);
+        if (st == 0) { ++it; continue; }
+        if (st == 2) {
+            it = s->pending.erase(it);
+            fail(s, r, r->status, r->err);
+            any = true;
+            continue;
+        }
+        if (s->active >= s->cfg.max_active) break;
+        size_t off;
+        if (!s->arena.alloc(r->total_bytes, &off)) break;   // the oldest waits for memory
+        r->off = off;
+        r->d = s->arena.base() + off;
+        Lane &L = s->lanes[r->lane];
+        const Plan &p = r->plan;
+        char *q = r->d + r->state_bytes;
+        hipError_t e = hipSuccess;
+        if (!p.rops.empty())
+            e = hipMemcpyAsync(q, p.rops.data(), p.rops.size() * sizeof(RegOp), hipMemcpyHostToDevice, L.stream);
+        q += align_up(p.rops.size() * sizeof(RegOp));
+        if (e == hipSuccess && !p.maps.empty())
+            e = hipMemcpyAsync(q, p.maps.data(), p.maps.size() * sizeof(PhaseMap), hipMemcpyHostToDevice, L.stream);
+        q += align_up(p.maps.size() * sizeof(PhaseMap));
+        if (e == hipSuccess && !p.mems.empty())
+            e = hipMemcpyAsync(q, p.mems.data(), p.mems.size() * sizeof(PassMem), hipMemcpyHostToDevice, L.stream);
+        if (e == hipSuccess)
+            e = launch_set_basis(s->cfg.dtype, r->d, uint64_t(1) << r->qubits, r->basis, L.stream);
+        it = s->pending.erase(it);
+        any = true;
+        ++s->active;
+        r->t_admit = now_ms(s->t0);
+        if (e != hipSuccess) {
+            // the copies may still reference the block: let the stream drain first
+            hipStreamSynchronize(L.stream);
+            s->arena.release(r->off, r->total_bytes);
+            --s->active;
+            fail(s, r, HGM_E_HIP, hip_msg(e, "request upload"));
+            continue;
+        }
+        L.active.push_back(r);
+    }
+    return any;
+}
+
+RegOp *dev_rops(Req *r) { return reinterpret_cast<RegOp *>(r->d + r->state_bytes); }
+PhaseMap *dev_maps(Req *r) {
+    return reinterpret_cast<PhaseMap *>(r->d + r->state_bytes + align_up(r->plan.rops.size() * sizeof(RegOp)));
+}
+PassMem *dev_mems(Req *r) {
+    return reinterpret_cast<PassMem *>(r->d + r->state_bytes + align_up(r->plan.rops.size() * sizeof(RegOp)) +
+                                       align_up(r->plan.maps.size() * sizeof(PhaseMap)));
+}
+ReadoutPartial *dev_partials(Req *r) {
+    return reinterpret_cast<ReadoutPartial *>(r->d + r->state_bytes + r->prog_bytes);
+}
+
+// Launches the next pass of every active state of the lane as one step.
+bool launch_step(hgm_server *s, Lane &L) {
+    if (L.active.empty() || (int)L.inflight.size() >= kDepth) return false;
+    const int slot = L.next_slot;
+    L.next_slot = (L.next_slot + 1) % kDepth;
+    char *h = L.h_ring + (size_t)slot * s->slot_bytes;
+    char *d = L.d_ring + (size_t)slot * s->slot_bytes;
+    BatchItem *items = reinterpret_cast<BatchItem *>(h);
+    ReadoutTask *tasks = reinterpret_cast<ReadoutTask *>(h + s->tasks_off);
+    ReadoutPartial *d_results = reinterpret_cast<ReadoutPartial *>(d + s->results_off);
+
+    std::vector<Req *> order(L.active.begin(), L.active.end());
+    std::stable_sort(order.begin(), order.end(), [](const Req *a, const Req *b) {
+        if (a->plan.shape.R != b->plan.shape.R) return a->plan.shape.R < b->plan.shape.R;
+        return a->plan.shape.B < b->plan.shape.B;
+    });
+    struct Group { int A, B; int first, count; int64_t tiles; size_t lds; };
+    std::vector<Group> groups;
+    Step step;
+    step.slot = slot;
+    int n_tasks = 0;
+    for (int i = 0; i < (int)order.size(); ++i) {
+        Req *r = order[i];
+        const int A = 2 * r->plan.shape.R, B = r->plan.shape.B;
+        if (groups.empty() || groups.back().A != A || groups.back().B != B) groups.push_back({A, B, i, 0, 0, 0});
+        Group &g = groups.back();
+        const Pass &ps = r->plan.passes[r->passes[r->next]];
+        const bool last = r->next + 1 == r->passes.size();
+        BatchItem &it = items[i];
+        it.state = r->d;
+        it.ops = dev_rops(r) + ps.rop_begin;
+        it.maps = dev_maps(r);
+        it.pm = dev_mems(r) + ps.mem;
+        it.nonlocal_mask = ((uint64_t(1) << r->qubits) - 1) & ~ps.local_mask;
+        it.tile_begin = g.tiles;
+        it.partials = last ? dev_partials(r) : nullptr;
+        it.n_ops = (int32_t)ps.rop_count;
+        it.pad = 0;
+        g.count++;
+        g.tiles += r->n_tiles;
+        if (ps.phases > 1)
+            g.lds = std::max(g.lds, (size_t(1) << r->plan.shape.r) * amp_bytes(s->cfg.dtype));
+        if (last) {
+            tasks[n_tasks].partials = dev_partials(r);
+            tasks[n_tasks].n = r->n_tiles;
+            tasks[n_tasks].out = d_results + n_tasks;
+            ++n_tasks;
+            step.finishing.push_back(r);
+        }
+        r->next++;
+    }
+    hipError_t e = hipMemcpyAsync(d, h, order.size() * sizeof(BatchItem), hipMemcpyHostToDevice, L.stream);
+    if (e == hipSuccess && n_tasks)
+        e = hipMemcpyAsync(d + s->tasks_off, h + s->tasks_off, n_tasks * sizeof(ReadoutTask), hipMemcpyHostToDevice,
+                           L.stream);
+    for (const Group &g : groups) {
+        if (e != hipSuccess) break;
+        e = launch_batch_reg(s->cfg.dtype, g.A, g.B, reinterpret_cast<const BatchItem *>(d) + g.first, g.count,
+                             g.tiles, g.lds, L.stream);
+    }
+    if (e == hipSuccess && n_tasks)
+        e = launch_readout_final(reinterpret_cast<const ReadoutTask *>(d + s->tasks_off), n_tasks, L.stream);
+    if (e == hipSuccess && n_tasks)
+        e = hipMemcpyAsync(h + s->results_off, d + s->results_off, n_tasks * sizeof(ReadoutPartial),
+                           hipMemcpyDeviceToHost, L.stream);
+    for (Req *r : step.finishing) {
+        if (e != hipSuccess) break;
+        if (r->state_out)
+            e = hipMemcpyAsync(r->state_out, r->d, (size_t(1) << r->qubits) * amp_bytes(s->cfg.dtype),
+                               hipMemcpyDeviceToHost, L.stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(L.ev[slot], L.stream);
+    if (e != hipSuccess) {
+        // a failed launch leaves the lane's states undefined: fail all of them
+        hipStreamSynchronize(L.stream);
+        const std::string msg = hip_msg(e, "serving step");
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            if (s->fatal.empty()) s->fatal = msg;
+        }
+        for (Req *r : L.active) {
+            s->arena.release(r->off, r->total_bytes);
+            --s->active;
+            fail(s, r, HGM_E_HIP, msg);
+        }
+        L.active.clear();
+        return true;
+    }
+    L.active.erase(std::remove_if(L.active.begin(), L.active.end(),
+                                  [](const Req *r) { return r->next == r->passes.size(); }),
+                   L.active.end());
+    L.inflight.push_back(std::move(step));
+    return true;
+}
+
+bool complete_steps(hgm_server *s, Lane &L) {
+    bool any = false;
+    while (!L.inflight.empty()) {
+        Step &st = L.inflight.front();
+        const hipError_t q = hipEventQuery(L.ev[st.slot]);
+        if (q == hipErrorNotReady) break;
+        const ReadoutPartial *res =
+            reinterpret_cast<const ReadoutPartial *>(L.h_ring + (size_t)st.slot * s->slot_bytes + s->results_off);
+        const double t = now_ms(s->t0);
+        for (size_t k = 0; k < st.finishing.size(); ++k) {
+            Req *r = st.finishing[k];
+            s->arena.release(r->off, r->total_bytes);
+            --s->active;
+            if (q != hipSuccess) {
+                fail(s, r, HGM_E_HIP, hip_msg(q, "serving step"));
+                continue;
+            }
+            r->result = res[k];
+            r->t_done = t;
+            finish(s, r);
+        }
+        L.inflight.pop_front();
+        any = true;
+    }
+    return any;
+}
+
+void worker_loop(hgm_server *s) {
+    hipSetDevice(s->device);
+    for (;;) {
+        bool stopping;
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            while (!s->submitted.empty()) {
+                s->pending.push_back(s->submitted.front());
+                s->submitted.pop_front();
+            }
+            stopping = s->stop;
+        }
+        bool progress = false;
+        for (Lane &L : s->lanes) progress |= complete_steps(s, L);
+        progress |= admit(s);
+        for (Lane &L : s->lanes) progress |= launch_step(s, L);
+        bool idle = s->pending.empty();
+        for (Lane &L : s->lanes) idle = idle && L.active.empty() && L.inflight.empty();
+        if (stopping && idle) return;
+        if (progress) continue;
+        if (idle) {
+            std::unique_lock<std::mutex> g(s->mu);
+            s->cv_work.wait_for(g, std::chrono::milliseconds(5),
+                                [&] { return s->stop || !s->submitted.empty(); });
+        } else {
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
+    }
+}
+
+void release_lanes(hgm_server *s) {
+    for (Lane &L : s->lanes) {
+        if (L.stream) hipStreamSynchronize(L.stream);
+        for (hipEvent_t &e : L.ev)
+            if (e) { hipEventDestroy(e); e = nullptr; }
+        if (L.d_ring) hipFree(L.d_ring);
+        if (L.h_ring) hipHostFree(L.h_ring);
+        if (L.stream) hipStreamDestroy(L.stream);
+        L.d_ring = L.h_ring = nullptr;
+        L.stream = nullptr;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void hgm_server_default_config(hgm_server_config *c) {
+    std::memset(c, 0, sizeof *c);
+    c->dtype = HGM_C64;
+    c->max_active = 256;
+    c->hbm_budget_bytes = uint64_t(32) << 30;
+    c->planner_threads = 4;
+    c->autotune = 0;
+    c->min_qubits = 10;
+    c->max_qubits = 26;
+}
+
+int hgm_server_create(const hgm_server_config *cfg, hgm_server **out) {
+    if (!cfg || !out) { set_error("null argument"); return HGM_E_ARG; }
+    if (cfg->dtype != HGM_C64 && cfg->dtype != HGM_C128) { set_error("unknown amplitude dtype"); return HGM_E_ARG; }
+    if (cfg->max_active < 1 || cfg->max_active > 65536 || cfg->planner_threads < 1 || cfg->planner_threads > 64 ||
+        cfg->hbm_budget_bytes < (uint64_t(1) << 20)) {
+        set_error("server config out of range (max_active 1..65536, planner_threads 1..64, budget >= 1 MiB)");
+        return HGM_E_ARG;
+    }
+    if (cfg->autotune && (cfg->min_qubits < 1 || cfg->max_qubits > kMaxQubits || cfg->min_qubits > cfg->max_qubits)) {
+        set_error("autotune qubit range out of range");
+        return HGM_E_ARG;
+    }
+    int rc = hgm_init(-1);
+    if (rc) return rc;
+    auto s = std::make_unique<hgm_server>();
+    s->cfg = *cfg;
+    hipGetDevice(&s->device);
+    s->t0 = std::chrono::steady_clock::now();
+    s->tune.resize(kMaxQubits + 1);
+    for (int n = 1; n <= kMaxQubits; ++n) s->tune[n] = serving_default_tuning(n);
+    if (cfg->autotune) {
+        for (int n = std::max(cfg->min_qubits, 14); n <= cfg->max_qubits; ++n) {
+            const std::vector<hgm_op> ops = timing_circuit(n, 4);
+            hgm_tuning best;
+            double ms;
+            if ((rc = hgm_autotune(n, cfg->dtype, ops.data(), (int64_t)ops.size(), 1, HGM_KERNEL_REG, &best, &ms)))
+                return rc;
+            best.ops_block = 1 << 30;
+            s->tune[n] = best;
+        }
+    }
+    std::string err;
+    if ((rc = s->arena.init(cfg->hbm_budget_bytes, &err))) { set_error(err); return rc; }
+    const size_t m = (size_t)cfg->max_active;
+    s->tasks_off = align_up(m * sizeof(BatchItem));
+    s->results_off = s->tasks_off + align_up(m * sizeof(ReadoutTask));
+    s->slot_bytes = s->results_off + align_up(m * sizeof(ReadoutPartial));
+    for (Lane &L : s->lanes) {
+        hipError_t e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking);
+        for (hipEvent_t &ev : L.ev)
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipMalloc(&L.d_ring, kDepth * s->slot_bytes);
+        if (e == hipSuccess) e = hipHostMalloc(&L.h_ring, kDepth * s->slot_bytes, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            set_error(hip_msg(e, "serving lane set-up"));
+            release_lanes(s.get());
+            s->arena.destroy();
+            return HGM_E_HIP;
+        }
+    }
+    hgm_server *p = s.release();
+    for (int i = 0; i < cfg->planner_threads; ++i) p->planners.emplace_back(planner_loop, p);
+    p->worker = std::thread(worker_loop, p);
+    *out = p;
+    return HGM_OK;
+}
+
+int hgm_server_submit(hgm_server *s, const hgm_request *rq, int64_t *id) {
+    if (!s || !rq) { set_error("null argument"); return HGM_E_ARG; }
+    if (rq->qubits < 1 || rq->qubits > kMaxQubits) { set_error("qubits out of range"); return HGM_E_ARG; }
+    if (rq->n_ops < 1 || !rq->ops || rq->iterations < 1) {
+        set_error("a request needs at least one gate and one iteration");
+        return HGM_E_ARG;
+    }
+    if (rq->basis >> rq->qubits) { set_error("basis index outside the state"); return HGM_E_ARG; }
+    Req *r = new Req;
+    r->qubits = rq->qubits;
+    r->iterations = rq->iterations;
+    r->basis = rq->basis;
+    r->state_out = rq->state_out;
+    r->ops.assign(rq->ops, rq->ops + rq->n_ops);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stop) { delete r; set_error("server is shutting down"); return HGM_E_STATE; }
+        r->id = s->next_id++;
+        r->t_submit = now_ms(s->t0);
+        ++s->n_submitted;
+        s->plan_q.push_back(r);
+        s->submitted.push_back(r);
+    }
+    if (id) *id = r->id;
+    s->cv_plan.notify_one();
+    s->cv_work.notify_all();
+    return HGM_OK;
+}
+
+int hgm_server_poll(hgm_server *s, hgm_result *out, int cap, int *n, int timeout_ms) {
+    if (!s || !n || cap < 0 || (cap > 0 && !out)) { set_error("bad argument"); return HGM_E_ARG; }
+    std::unique_lock<std::mutex> g(s->mu);
+    if (s->done.empty() && timeout_ms > 0)
+        s->cv_done.wait_for(g, std::chrono::milliseconds(timeout_ms), [&] { return !s->done.empty(); });
+    int k = 0;
+    while (k < cap && !s->done.empty()) {
+        out[k++] = s->done.front();
+        s->done.pop_front();
+    }
+    *n = k;
+    return HGM_OK;
+}
+
+int hgm_server_wait_all(hgm_server *s) {
+    if (!s) { set_error("null argument"); return HGM_E_ARG; }
+    std::unique_lock<std::mutex> g(s->mu);
+    s->cv_done.wait(g, [&] { return s->n_done == s->n_submitted; });
+    if (!s->fatal.empty()) { set_error(s->fatal); return HGM_E_HIP; }
+    return HGM_OK;
+}
+
+int hgm_server_tuning(const hgm_server *s, int qubits, hgm_tuning *out) {
+    if (!s || !out || qubits < 1 || qubits > kMaxQubits) { set_error("bad argument"); return HGM_E_ARG; }
+    *out = s->tune[qubits];
+    return HGM_OK;
+}
+
+double hgm_server_clock_ms(const hgm_server *s) { return s ? now_ms(s->t0) : 0.0; }
+
+void hgm_server_destroy(hgm_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv_plan.notify_all();
+    s->cv_work.notify_all();
+    for (std::thread &t : s->planners) t.join();
+    s->worker.join();
+    release_lanes(s);
+    s->arena.destroy();
+    delete s;
+}
+
+int hgm_autotune(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, int kernel,
+                 hgm_tuning *best, double *best_ms) {
+    if (!best || (n_ops > 0 && !ops) || iterations < 1) { set_error("bad argument"); return HGM_E_ARG; }
+    if (kernel != HGM_KERNEL_AUTO && kernel != HGM_KERNEL_REG && kernel != HGM_KERNEL_LDS) {
+        set_error("the autotuner times the AUTO, REG and LDS kernel families");
+        return HGM_E_ARG;
+    }
+    hgm_state *st = nullptr;
+    int rc = hgm_state_create(qubits, dtype, &st);
+    if (rc) return rc;
+    double best_t = 1e300;
+    bool found = false;
+    for (int A : {8, 16, 32}) {
+        for (int r = 8; r <= 13; ++r) {
+            const int64_t B = (int64_t(1) << r) / A;
+            if (B < 64 || B > 1024 || (kernel != HGM_KERNEL_LDS && B > reg_kernel_max_threads(A))) continue;
+            for (int c = 2; c <= 4; ++c) {
+                hgm_tuning t;
+                hgm_default_tuning(&t);
+                t.block_size = (int)B;
+                t.repeat_count = A / 2;
+                t.gpu_region_bits = r;
+                t.gpu_coalesced_bits = c;
+                t.kernel = kernel;
+                t.ops_block = 1 << 30;
+                if (hgm_validate_tuning(qubits, dtype, &t) != HGM_OK) continue;
+                hgm_plan *p = nullptr;
+                if (hgm_plan_build(qubits, dtype, ops, n_ops, iterations, &t, &p) != HGM_OK) continue;
+                double ms = 1e300;
+                hgm_state_set_basis(st, 0);
+                rc = hgm_state_run_plan(st, p, nullptr);   // warm-up
+                for (int rep = 0; rep < 2 && rc == HGM_OK; ++rep) {
+                    hgm_stats stats{};
+                    rc = hgm_state_run_plan(st, p, &stats);
+                    ms = std::min(ms, stats.kernel_ms);
+                }
+                hgm_plan_destroy(p);
+                if (rc) { hgm_state_destroy(st); return rc; }
+                if (ms < best_t) { best_t = ms; *best = t; found = true; }
+            }
+        }
+    }
+    hgm_state_destroy(st);
+    if (!found) { set_error("no valid tile shape for this state"); return HGM_E_TUNING; }
+    if (best_ms) *best_ms = best_t;
+    return HGM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,114 @@
+"""The batched serving engine on the GPU: every result is the same alone and
+in the mix, FP32 fidelity against FP64 >= 1 - 1e-6, FP64 results within 1e-12
+of the oracle, readouts exact, admission by HBM footprint."""
+import sys
+
+import numpy as np
+import pytest
+
+from simulador_quantico_amd import _native as N
+from simulador_quantico_amd import serving as S
+from tests.test_serving_mix import oracle_run_job
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def small_mix(count, seed, qmin, qmax, grover_max_iter=6):
+    return S.make_mix(count, rate=1000.0, seed=seed, qmin=qmin, qmax=qmax, grover_max_iter=grover_max_iter)
+
+
+def run_collect(srv, jobs, dtype, together):
+    """Final states and readouts of every job, submitted all at once or one by one."""
+    states = {j.index: np.zeros(1 << j.qubits, dtype) for j in jobs}
+    res = {}
+    if together:
+        ids = {srv.submit(j.qubits, j.ops, j.basis, state_out=states[j.index]): j.index for j in jobs}
+        srv.wait_all()
+        for r in srv.poll(cap=len(jobs) + 8):
+            res[ids[r["id"]]] = r
+    else:
+        for j in jobs:
+            rid = srv.submit(j.qubits, j.ops, j.basis, state_out=states[j.index])
+            srv.wait_all()
+            (r,) = srv.poll(cap=4)
+            assert r["id"] == rid
+            res[j.index] = r
+    assert len(res) == len(jobs)
+    return states, res
+
+
+def test_served_results_same_alone_and_in_mix_fp32():
+    jobs = small_mix(28, seed=5, qmin=10, qmax=17)
+    say("serving alone vs mix:", [(j.kind, j.qubits) for j in jobs])
+    with S.Server(np.complex64, max_active=64, hbm_budget_gib=2) as srv:
+        alone, ra = run_collect(srv, jobs, np.complex64, together=False)
+        mixed, rm = run_collect(srv, jobs, np.complex64, together=True)
+    for j in jobs:
+        a, m = alone[j.index], mixed[j.index]
+        assert ra[j.index]["status"] == 0 and rm[j.index]["status"] == 0
+        # same kernels and programs for a state whatever runs beside it: tolerance 1e-7 max-abs (expected 0)
+        assert np.abs(a - m).max() <= 1e-7, (j.kind, j.qubits)
+        assert ra[j.index]["argmax"] == rm[j.index]["argmax"]
+        assert abs(ra[j.index]["norm2"] - rm[j.index]["norm2"]) <= 1e-12
+        ref = oracle_run_job(j)
+        f = S.fidelity(ref, m)
+        assert f >= 1 - 1e-6, (j.kind, j.qubits, f)
+        p = np.abs(m.astype(np.complex128)) ** 2
+        assert abs(rm[j.index]["norm2"] - p.sum()) < 1e-9
+        assert rm[j.index]["argmax"] == int(np.argmax(p))
+        if j.kind == "adder":
+            assert rm[j.index]["argmax"] == j.expect
+
+
+def test_served_fp64_matches_oracle():
+    jobs = small_mix(10, seed=8, qmin=10, qmax=15)
+    with S.Server(np.complex128, max_active=16, hbm_budget_gib=1) as srv:
+        states, res = run_collect(srv, jobs, np.complex128, together=True)
+    for j in jobs:
+        ref = oracle_run_job(j)
+        assert np.abs(states[j.index] - ref).max() < 1e-12, (j.kind, j.qubits)
+        pr = np.abs(ref) ** 2
+        r = res[j.index]
+        assert abs(r["norm2"] - 1) < 1e-12 and abs(r["p_max"] - pr.max()) < 1e-12
+        assert pr[r["argmax"]] >= pr.max() - 1e-12
+
+
+def test_admission_by_footprint_and_errors():
+    n = 20                                 # 8 MiB per FP32 state
+    jobs = [S.build_job(i, 0.0, "hadamard", n, np.random.default_rng(i)) for i in range(12)]
+    with S.Server(np.complex64, max_active=64, hbm_budget_gib=48 / 1024) as srv:
+        ids = [srv.submit(j.qubits, j.ops) for j in jobs]
+        big = S.build_job(99, 0.0, "hadamard", 23, np.random.default_rng(0))   # 64 MiB > budget
+        bid = srv.submit(big.qubits, big.ops)
+        srv.wait_all()
+        res = {r["id"]: r for r in srv.poll(cap=64)}
+        with pytest.raises(N.HGMError):
+            srv.submit(n, jobs[0].ops[:0])
+    assert res[bid]["status"] == 1        # HGM_E_ARG: can never fit
+    admits = sorted(res[i]["t_admit_ms"] for i in ids)
+    dones = sorted(res[i]["t_done_ms"] for i in ids)
+    for i in ids:
+        r = res[i]
+        assert r["status"] == 0 and r["argmax"] == 0 and abs(r["p_max"] - 2.0 ** -n) < 1e-9
+        assert abs(r["norm2"] - 1) < 1e-5
+    # at most 5 states (8 MiB + programs + partials) fit in 48 MiB: the 6th admission waits for a completion
+    assert admits[5] >= dones[0]
+
+
+def test_autotune_picks_a_valid_shape():
+    import ctypes as C
+    n = 18
+    job = S.build_job(0, 0.0, "random", n, np.random.default_rng(1))
+    best = N.Tuning()
+    ms = C.c_double()
+    N.check(N.lib.hgm_autotune(n, N.HGM_C64, S._op_ptr(job.ops), len(job.ops), 1, N.HGM_KERNEL_REG,
+                               C.byref(best), C.byref(ms)))
+    say("autotune", n, best.block_size, best.repeat_count, best.gpu_region_bits, best.gpu_coalesced_bits, ms.value)
+    assert ms.value > 0
+    assert 2 * best.block_size * best.repeat_count == 1 << best.gpu_region_bits
+    assert N.lib.hgm_validate_tuning(n, N.HGM_C64, C.byref(best)) == 0
