@@ -29,16 +29,42 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3
+
+
+def algorithmic_flops(ops, n: int, iterations: int) -> float:
+    """Flops of one execution of the gate list, counted as the arithmetic the
+    reference's kernel writes (src/core/kernel.cu:109-111): a dense gate does
+    4 complex multiplies and 2 complex adds per amplitude pair (14 flops per
+    amplitude); a diagonal or anti-diagonal gate one complex multiply (6 flops)
+    per amplitude it changes; controls restrict both to the matching amplitudes."""
+    total = 0.0
+    for i in range(len(ops)):
+        o = ops[i]
+        m = o.m
+        frac = 0.5 ** bin(int(o.control_mask)).count("1")
+        if m[2] == 0 and m[3] == 0 and m[4] == 0 and m[5] == 0:
+            # diagonal: a multiply where the factor is not exactly 1
+            per = (0 if (m[0] == 1 and m[1] == 0) else 3) + (0 if (m[6] == 1 and m[7] == 0) else 3)
+        elif m[0] == 0 and m[1] == 0 and m[6] == 0 and m[7] == 0:
+            per = 6
+        else:
+            per = 14
+        total += per * frac * float(1 << n)
+    return total * iterations
 METRIC = "amp-updates/s + %HBM roofline @30q; circuits/s and p99 latency on serving mix"
 
 WORKLOADS = {
     # name: (qubits, dtype, builder, tuning)
     "random30": dict(qubits=30, dtype="c128", kind="random", depth=20, seed=1,
-                     tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=2)),
+                     tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
+                                 kernel=3)),
     "u3layers30": dict(qubits=30, dtype="c128", kind="u3layers", depth=20, seed=1,
                        tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=2)),
     "random32f": dict(qubits=32, dtype="c64", kind="random", depth=20, seed=1,
-                      tuning=dict(block_size=256, repeat_count=32, gpu_region_bits=14, gpu_coalesced_bits=3)),
+                      tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
+                                  kernel=3)),
     "hadamard20": dict(qubits=20, dtype="c128", kind="hadamard", layers=1,
                        tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4)),
     "grover22": dict(qubits=22, dtype="c128", kind="grover", seed=22,
@@ -49,6 +75,21 @@ WORKLOADS = {
 
 
 KERNEL_NAMES = {1: "k_pass_lds", 2: "k_pass_reg", 3: "hgm_pass_* (plan-compiled)"}
+
+
+def measured_traffic(workload: str, tun: dict):
+    """HBM bytes per pass-kernel launch measured by the FETCH_SIZE / WRITE_SIZE
+    counters (rocprofv3 --pmc, one run each, scaled by the same counters on a
+    plain copy of known size), stored beside the profile they come from; None
+    when no measurement exists for this workload and tile shape."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    if rec.get("tile") != {k: int(v) for k, v in tun.items()}:
+        return None
+    return rec
 
 
 def build_circuit(w):
@@ -124,6 +165,38 @@ def cpu_baseline(max_seconds: float = 20.0):
     return {"value": rate, "unit": "amp-updates/s", "cores": threads, "kind": kind,
             "sample": f"t_PAR_CPU (FP32, cpu_region_bits=14, cpu_coalesced_bits=11) on one random layer "
                       f"({gates} gates) at {n} qubits, {dt:.1f} s, {threads} threads"}
+
+
+def fidelity_report(names=("hadamard20", "grover22", "draper29", "random30", "random32f")) -> dict:
+    """FP32 states against FP64 states of the same circuits, both on the GPU
+    (the FP64 path is the one checked against the FP64 oracle within 1e-12):
+    fidelity, max amplitude error, kernel times."""
+    import simulador_quantico_amd as hq
+    from simulador_quantico_amd import _native as N
+    out = {}
+    for name in names:
+        w = WORKLOADS[name]
+        n = w["qubits"]
+        d, ops, iterations, basis, title = build_circuit(w)
+        row = {"circuit": title}
+        states = {}
+        for dt, tun in ((np.complex128, dict(block_size=256, repeat_count=4, gpu_region_bits=11,
+                                               gpu_coalesced_bits=2, kernel=2)),
+                        (np.complex64, dict(block_size=256, repeat_count=8, gpu_region_bits=12,
+                                              gpu_coalesced_bits=3, kernel=2))):
+            s = hq.State(n, dt)
+            s.set_basis(basis)
+            st = s.apply(ops, N.default_tuning(**tun), iterations=iterations)
+            row["kernel_ms_" + ("f64" if dt == np.complex128 else "f32")] = st["kernel_ms"]
+            states[dt] = s
+        f, m = states[np.complex128].compare(states[np.complex64])
+        i64, p64 = states[np.complex128].argmax()
+        i32, p32 = states[np.complex64].argmax()
+        row.update(fidelity=f, max_abs_err=m, argmax_equal=bool(i64 == i32), p_max_f64=p64, p_max_f32=p32)
+        del states
+        print(f"bench: fidelity {name}: {row}", file=sys.stderr, flush=True)
+        out[name] = row
+    return out
 
 
 def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026, autotune: bool = False,
@@ -203,11 +276,15 @@ def main():
     ap.add_argument("--tuning", default=None, help="B,R,r,c override")
     ap.add_argument("--kernel", type=int, default=None, help="0 auto, 1 LDS, 2 register, 3 plan-compiled")
     ap.add_argument("--serving", action="store_true", help="run only the serving mix and print its JSON line")
+    ap.add_argument("--fidelity", action="store_true",
+                    help="run only the FP32-vs-FP64 fidelity report of the benchmark circuits")
     ap.add_argument("--requests", type=int, default=4096)
     ap.add_argument("--rate", type=float, default=2000.0, help="serving mix arrival rate (requests/s)")
     ap.add_argument("--seq-sample", type=int, default=256,
                     help="requests of the same stream run by the sequential FP64 baseline")
     ap.add_argument("--autotune", action="store_true", help="serving: autotune tile shapes per qubit count")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg of the default run")
+    ap.add_argument("--no-fidelity", action="store_true", help="skip the FP32-vs-FP64 leg of the default run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain device copy of the workload's state (the known-size reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters) instead of the circuit")
@@ -219,6 +296,10 @@ def main():
 
     import simulador_quantico_amd as hq
     from simulador_quantico_amd import _native as N
+
+    if args.fidelity:
+        print(json.dumps({"fidelity_fp32_vs_fp64": fidelity_report()}))
+        return
 
     if args.serving:
         out = serving_bench(args.requests, args.rate, args.seq_sample, autotune=args.autotune)
@@ -281,6 +362,9 @@ def main():
     bytes_per_launch = 2.0 * (1 << n) * amp_bytes
     mean_launch_ms = kernel_ms / max(1, passes)
     achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
+    flops = algorithmic_flops(ops, n, iterations)
+    tflops = flops * args.steps / (kernel_ms * 1e-3) / 1e12
+    fpeak = FP64_PEAK_TFLOPS if dtype == np.complex128 else FP32_PEAK_TFLOPS
     norm = state.norm2()
 
     out = {
@@ -303,13 +387,34 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": KERNEL_NAMES.get(plan.kernel, "?"),
                      "bytes_per_launch": bytes_per_launch,
-                     "mean_launch_ms": mean_launch_ms},
+                     "mean_launch_ms": mean_launch_ms,
+                     "compute": {"achieved": tflops, "peak": fpeak, "unit": "TFLOP/s", "frac": tflops / fpeak,
+                                 "flops_per_step": flops,
+                                 "counting": "dense gate 14 flop/amplitude, (anti)diagonal 6 flop per changed "
+                                             "amplitude, over all passes' kernel time"}},
     }
+    traffic = measured_traffic(args.workload, tun)
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic["traffic_bytes_per_launch"]
+        out["roofline"]["traffic_source"] = traffic["source"]
+    default_run = args.tuning is None and args.kernel is None and args.planner is None and args.ops_block is None
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline()
         except Exception as e:  # the baseline is reported, never the target
             out["cpu_baseline"] = {"value": None, "error": str(e)}
+    del plan, state
+    if default_run and not args.no_serving:
+        try:
+            out["serving"] = serving_bench(args.requests, args.rate, min(args.seq_sample, 128),
+                                           autotune=args.autotune)
+        except Exception as e:
+            out["serving"] = {"error": str(e)}
+    if default_run and not args.no_fidelity:
+        try:
+            out["fidelity_fp32_vs_fp64"] = fidelity_report()
+        except Exception as e:
+            out["fidelity_fp32_vs_fp64"] = {"error": str(e)}
     print(json.dumps(out))
 
 

@@ -162,6 +162,11 @@ int hgm_state_collapse_reset(hgm_state *s, int qubit, int outcome, double p);
 /* Index of the largest |a|^2 (ties -> lowest index) and its probability. */
 int hgm_state_argmax(const hgm_state *s, uint64_t *index, double *prob);
 
+/* Fidelity |<a|b>|^2 / (<a|a><b|b>) and max |a_i - b_i| of two states of the
+ * same qubit count and any amplitude types (FP64 accumulation on the device):
+ * the FP32-against-FP64 check of large states without a host copy. */
+int hgm_state_compare(const hgm_state *a, const hgm_state *b, double *fidelity, double *max_abs_err);
+
 /* ------------------------------------------------------------------ plans (CPU only) */
 typedef struct hgm_plan hgm_plan;
 int hgm_plan_build(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations,

@@ -101,6 +101,7 @@ SIGNATURES = [
     ("hgm_state_prob_one", C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
     ("hgm_state_collapse_reset", C.c_int, [_P, C.c_int, C.c_int, C.c_double]),
     ("hgm_state_argmax", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+    ("hgm_state_compare", C.c_int, [_P, _P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("hgm_plan_build", C.c_int, [C.c_int, C.c_int, C.POINTER(Op), C.c_int64, C.c_int, C.POINTER(Tuning),
                                  C.POINTER(_P)]),
     ("hgm_plan_destroy", None, [_P]),

@@ -511,6 +511,35 @@ __global__ void __launch_bounds__(256) k_argmax(const typename Vec<T>::type *sta
     if (threadIdx.x == 0) { pmax[blockIdx.x] = sm[0]; pidx[blockIdx.x] = si[0]; }
 }
 
+// Per-block partials of the comparison of two states of any amplitude types:
+// Re and Im of sum conj(a)*b, sum |a|^2, sum |b|^2 (FP64), and max |a - b|.
+template <class TA, class TB>
+__global__ void __launch_bounds__(256) k_compare(const typename Vec<TA>::type *a, const typename Vec<TB>::type *b,
+                                                  uint64_t n, double *partial) {
+    __shared__ double sh[5][256];
+    double re = 0, im = 0, na = 0, nb = 0, mx = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const double ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y;
+        re += ar * br + ai * bi;
+        im += ar * bi - ai * br;
+        na += ar * ar + ai * ai;
+        nb += br * br + bi * bi;
+        const double dr = ar - br, di = ai - bi;
+        mx = fmax(mx, sqrt(dr * dr + di * di));
+    }
+    const int t = threadIdx.x;
+    sh[0][t] = re; sh[1][t] = im; sh[2][t] = na; sh[3][t] = nb; sh[4][t] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int k = 0; k < 4; ++k) sh[k][t] += sh[k][t + w];
+            sh[4][t] = fmax(sh[4][t], sh[4][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t < 5) partial[5 * blockIdx.x + t] = sh[t][0];
+}
+
 template <class T>
 __global__ void k_collapse(typename Vec<T>::type *state, uint64_t half, int bit, int outcome, double norm) {
     using V = typename Vec<T>::type;
@@ -644,6 +673,20 @@ hipError_t launch_argmax(int dtype, const void *state, uint64_t n, double *pmax,
         hipLaunchKernelGGL(k_argmax<float>, dim3(blocks), dim3(256), 0, s, (const float2 *)state, n, pmax, pidx);
     else
         hipLaunchKernelGGL(k_argmax<double>, dim3(blocks), dim3(256), 0, s, (const double2 *)state, n, pmax, pidx);
+    return hipGetLastError();
+}
+
+hipError_t launch_compare(int dtype_a, const void *a, int dtype_b, const void *b, uint64_t n, double *partial,
+                          int blocks, hipStream_t s) {
+    const bool fa = dtype_a == HGM_C64, fb = dtype_b == HGM_C64;
+    if (fa && fb)
+        hipLaunchKernelGGL((k_compare<float, float>), dim3(blocks), dim3(256), 0, s, (const float2 *)a, (const float2 *)b, n, partial);
+    else if (fa)
+        hipLaunchKernelGGL((k_compare<float, double>), dim3(blocks), dim3(256), 0, s, (const float2 *)a, (const double2 *)b, n, partial);
+    else if (fb)
+        hipLaunchKernelGGL((k_compare<double, float>), dim3(blocks), dim3(256), 0, s, (const double2 *)a, (const float2 *)b, n, partial);
+    else
+        hipLaunchKernelGGL((k_compare<double, double>), dim3(blocks), dim3(256), 0, s, (const double2 *)a, (const double2 *)b, n, partial);
     return hipGetLastError();
 }
 

@@ -81,6 +81,11 @@ hipError_t launch_prob_halves(int dtype, const void *state, uint64_t n_amps, int
 hipError_t launch_argmax(int dtype, const void *state, uint64_t n_amps, double *pmax, uint64_t *pidx, int blocks,
                          hipStream_t s);
 
+// Per-block partials (5 doubles per block) of the comparison of two states of
+// n amplitudes: Re, Im of sum conj(a)*b, sum |a|^2, sum |b|^2, max |a - b|.
+hipError_t launch_compare(int dtype_a, const void *a, int dtype_b, const void *b, uint64_t n, double *partial,
+                          int blocks, hipStream_t s);
+
 // Collapse `bit` to `outcome`, moving the kept half to the bit=0 slot divided
 // by norm and zeroing the bit=1 slot (src/core/dgm_core.cpp:115-122).
 hipError_t launch_collapse(int dtype, void *state, uint64_t n_amps, int bit, int outcome, double norm, hipStream_t s);

@@ -222,33 +222,50 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
     } else {
         // Greedy: scan ahead from the oldest unscheduled gate; take every gate
         // that commutes with all the gates skipped before it and whose target
-        // is local or can still be made local.
+        // is local. The local set is either grown on demand (the first r-c
+        // new targets met) or fixed to a window of r-c contiguous bits above
+        // the coalesced ones; every window is tried and the set that admits
+        // the most gates wins (on layered circuits the windows tile the
+        // light cone better than first-come targets do).
         std::vector<char> done((size_t)N, 0);
+        std::vector<int64_t> cand;
         int64_t head = 0;
-        while (head < N) {
-            chosen.clear();
-            uint64_t local = low;
-            int nlocal = c;
+        auto scan = [&](uint64_t fixed, bool grow, uint64_t *local_out, std::vector<int64_t> &out) {
+            out.clear();
+            uint64_t local = grow ? low : fixed;
+            int nlocal = __builtin_popcountll(local);
             uint64_t bw = 0, br = 0;   // writes / reads of the skipped gates
             int64_t scanned = 0;
-            for (int64_t k = head; k < N && (int64_t)chosen.size() < cap && scanned < window; ++k) {
+            for (int64_t k = head; k < N && (int64_t)out.size() < cap && scanned < window; ++k) {
                 if (done[k]) continue;
                 ++scanned;
                 const GateInfo &g = info[k % n_ops];
                 bool ok = !(g.writes & (bw | br)) && !(g.reads & bw);
                 if (ok && (g.need & ~local)) {
-                    if (nlocal < r) { local |= g.need; ++nlocal; }
+                    if (grow && nlocal < r) { local |= g.need; ++nlocal; }
                     else ok = false;
                 }
                 if (ok) {
-                    chosen.push_back(k);
-                    done[k] = 1;
+                    out.push_back(k);
                 } else {
                     bw |= g.writes;
                     br |= g.reads;
                     if (bw == all) break;
                 }
             }
+            *local_out = local;
+        };
+        const int e = r - c;
+        while (head < N) {
+            uint64_t local;
+            scan(0, true, &local, chosen);
+            for (int s0 = c; s0 + e <= qubits; ++s0) {
+                const uint64_t L = low | (((uint64_t(1) << e) - 1) << s0);
+                uint64_t l2;
+                scan(L, false, &l2, cand);
+                if (cand.size() > chosen.size()) { chosen.swap(cand); local = L; }
+            }
+            for (int64_t k : chosen) done[k] = 1;
             while (head < N && done[head]) ++head;
             Pass p;
             finish_pass(local, plan, p);

@@ -9,6 +9,7 @@
 // single synchronisation at the end.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -59,7 +60,7 @@ struct Ctx {
     size_t mems_cap = 0;
     void *cache = nullptr;      // device buffer of hgm_execute (grow-only)
     size_t cache_cap = 0;
-    double *partial = nullptr;  // reduction partials (device)
+    double *partial = nullptr;  // reduction partials (device, 8 doubles per block)
     uint64_t *pidx = nullptr;
     double *h_partial = nullptr;  // pinned host mirror
     uint64_t *h_pidx = nullptr;
@@ -116,9 +117,9 @@ int init_locked(int device) {
     HGM_HIP(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
     c.cus = prop.multiProcessorCount;
     if (!c.partial) {
-        HGM_HIP(hipMalloc(&c.partial, 2 * kRedBlocks * sizeof(double)), "hipMalloc(partials)");
+        HGM_HIP(hipMalloc(&c.partial, 8 * kRedBlocks * sizeof(double)), "hipMalloc(partials)");
         HGM_HIP(hipMalloc(&c.pidx, kRedBlocks * sizeof(uint64_t)), "hipMalloc(partials)");
-        HGM_HIP(hipHostMalloc(&c.h_partial, 2 * kRedBlocks * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+        HGM_HIP(hipHostMalloc(&c.h_partial, 8 * kRedBlocks * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
         HGM_HIP(hipHostMalloc(&c.h_pidx, kRedBlocks * sizeof(uint64_t), hipHostMallocDefault), "hipHostMalloc");
     }
     c.device = device;
@@ -502,6 +503,28 @@ int hgm_state_argmax(const hgm_state *s, uint64_t *index, double *prob) {
         if (c.h_partial[b] > best || (c.h_partial[b] == best && c.h_pidx[b] < bi)) { best = c.h_partial[b]; bi = c.h_pidx[b]; }
     *index = bi;
     if (prob) *prob = best;
+    return HGM_OK;
+}
+
+int hgm_state_compare(const hgm_state *a, const hgm_state *b, double *fidelity, double *max_abs_err) {
+    if (!a || !b) { t_err = "null argument"; return HGM_E_ARG; }
+    if (a->qubits != b->qubits) { t_err = "states of different qubit counts"; return HGM_E_ARG; }
+    Ctx &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    const uint64_t n = uint64_t(1) << a->qubits;
+    const int blocks = blocks_for(n);
+    HGM_HIP(launch_compare(a->dtype, a->d, b->dtype, b->d, n, c.partial, blocks, c.stream), "compare");
+    HGM_HIP(hipMemcpyAsync(c.h_partial, c.partial, 5 * blocks * sizeof(double), hipMemcpyDeviceToHost, c.stream),
+            "compare");
+    HGM_HIP(hipStreamSynchronize(c.stream), "compare");
+    double re = 0, im = 0, na = 0, nb = 0, mx = 0;
+    for (int k = 0; k < blocks; ++k) {
+        const double *q = c.h_partial + 5 * k;
+        re += q[0]; im += q[1]; na += q[2]; nb += q[3];
+        mx = std::max(mx, q[4]);
+    }
+    if (fidelity) *fidelity = (na > 0 && nb > 0) ? (re * re + im * im) / (na * nb) : 0.0;
+    if (max_abs_err) *max_abs_err = mx;
     return HGM_OK;
 }
 

@@ -39,19 +39,27 @@ size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 std::string hip_msg(hipError_t e, const char *what) { return std::string(what) + ": " + hipGetErrorString(e); }
 
-// First-fit allocator over one device allocation; freed ranges merge with
-// their neighbours.
+// First-fit allocator over one device (or pinned host) allocation; freed
+// ranges merge with their neighbours.
 class Arena {
   public:
-    int init(size_t bytes, std::string *err) {
-        hipError_t e = hipMalloc(&base_, bytes);
-        if (e != hipSuccess) { *err = hip_msg(e, "hipMalloc(serving arena)"); base_ = nullptr; return HGM_E_HIP; }
+    int init(size_t bytes, bool pinned_host, std::string *err) {
+        hipError_t e = pinned_host ? hipHostMalloc(&base_, bytes, hipHostMallocDefault) : hipMalloc(&base_, bytes);
+        if (e != hipSuccess) {
+            *err = hip_msg(e, pinned_host ? "hipHostMalloc(program staging)" : "hipMalloc(serving arena)");
+            base_ = nullptr;
+            return HGM_E_HIP;
+        }
+        pinned_ = pinned_host;
         size_ = bytes;
         free_[0] = bytes;
         return HGM_OK;
     }
     void destroy() {
-        if (base_) hipFree(base_);
+        if (base_) {
+            if (pinned_) hipHostFree(base_);
+            else hipFree(base_);
+        }
         base_ = nullptr;
         free_.clear();
     }
@@ -91,6 +99,7 @@ class Arena {
 
   private:
     char *base_ = nullptr;
+    bool pinned_ = false;
     size_t size_ = 0, used_ = 0;
     std::map<size_t, size_t> free_;
 };
@@ -111,7 +120,7 @@ struct Req {
     size_t next = 0;
     int lane = 0;
     int64_t n_tiles = 1;
-    size_t state_bytes = 0, prog_bytes = 0, part_bytes = 0, total_bytes = 0, off = 0;
+    size_t state_bytes = 0, prog_bytes = 0, part_bytes = 0, total_bytes = 0, off = 0, stage_off = 0;
     char *d = nullptr;             // device block: state | programs | maps | pass tables | partials
     ReadoutPartial result{};
 };
@@ -121,18 +130,18 @@ struct Step {
     int slot = 0;
 };
 
+constexpr int kLanes = 2;          // 0: single-tile states (whole circuit per pass), 1: multi-tile states
+constexpr int kDepth = 3;          // steps in flight per lane
+
 struct Lane {
     hipStream_t stream = nullptr;
     std::vector<Req *> active;
     std::deque<Step> inflight;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    char *d_ring = nullptr;        // 2 slots: items | tasks | results
+    hipEvent_t ev[kDepth] = {};
+    char *d_ring = nullptr;        // kDepth slots: items | tasks | results
     char *h_ring = nullptr;        // pinned mirror
     int next_slot = 0;
 };
-
-constexpr int kLanes = 2;          // 0: single-tile states (whole circuit per pass), 1: multi-tile states
-constexpr int kDepth = 2;          // steps in flight per lane
 
 double now_ms(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -197,7 +206,8 @@ struct hgm_server {
     hgm_server_config cfg{};
     int device = 0;
     std::chrono::steady_clock::time_point t0;
-    Arena arena;
+    Arena arena;                              // device: states, programs, partials
+    Arena staging;                            // pinned host: programs on their way to the device
     Lane lanes[kLanes];
     size_t slot_bytes = 0, tasks_off = 0, results_off = 0;
     std::vector<hgm_tuning> tune;             // per qubit count
@@ -280,7 +290,7 @@ void planner_loop(hgm_server *s) {
                             align_up(p.mems.size() * sizeof(PassMem));
             r->part_bytes = align_up((size_t)r->n_tiles * sizeof(ReadoutPartial));
             r->total_bytes = r->state_bytes + r->prog_bytes + r->part_bytes;
-            if (r->total_bytes > s->arena.size()) {
+            if (r->total_bytes > s->arena.size() || r->prog_bytes > s->staging.size()) {
                 rc = HGM_E_ARG;
                 err = "the request's footprint exceeds the serving HBM budget";
             }
@@ -290,6 +300,13 @@ void planner_loop(hgm_server *s) {
         r->planned.store(rc == HGM_OK ? 1 : 2, std::memory_order_release);
         s->cv_work.notify_all();
     }
+}
+
+// Returns a request's device and staging blocks (its work has completed).
+void release_req(hgm_server *s, Req *r) {
+    s->arena.release(r->off, r->total_bytes);
+    if (r->prog_bytes) s->staging.release(r->stage_off, r->prog_bytes);
+    --s->active;
 }
 
 // Admit planned requests in arrival order while memory and slots allow.
@@ -306,22 +323,26 @@ bool admit(hgm_server *s) {
             continue;
         }
         if (s->active >= s->cfg.max_active) break;
-        size_t off;
+        size_t off = 0, soff = 0;
         if (!s->arena.alloc(r->total_bytes, &off)) break;   // the oldest waits for memory
+        if (r->prog_bytes && !s->staging.alloc(r->prog_bytes, &soff)) {
+            s->arena.release(off, r->total_bytes);
+            break;
+        }
         r->off = off;
+        r->stage_off = soff;
         r->d = s->arena.base() + off;
         Lane &L = s->lanes[r->lane];
         const Plan &p = r->plan;
-        char *q = r->d + r->state_bytes;
-        hipError_t e = hipSuccess;
-        if (!p.rops.empty())
-            e = hipMemcpyAsync(q, p.rops.data(), p.rops.size() * sizeof(RegOp), hipMemcpyHostToDevice, L.stream);
+        // programs go through pinned memory, so that the copy is queued on the
+        // lane's stream without waiting for the steps already in flight
+        char *h = s->staging.base() + soff, *q = h;
+        if (!p.rops.empty()) std::memcpy(q, p.rops.data(), p.rops.size() * sizeof(RegOp));
         q += align_up(p.rops.size() * sizeof(RegOp));
-        if (e == hipSuccess && !p.maps.empty())
-            e = hipMemcpyAsync(q, p.maps.data(), p.maps.size() * sizeof(PhaseMap), hipMemcpyHostToDevice, L.stream);
+        if (!p.maps.empty()) std::memcpy(q, p.maps.data(), p.maps.size() * sizeof(PhaseMap));
         q += align_up(p.maps.size() * sizeof(PhaseMap));
-        if (e == hipSuccess && !p.mems.empty())
-            e = hipMemcpyAsync(q, p.mems.data(), p.mems.size() * sizeof(PassMem), hipMemcpyHostToDevice, L.stream);
+        if (!p.mems.empty()) std::memcpy(q, p.mems.data(), p.mems.size() * sizeof(PassMem));
+        hipError_t e = hipMemcpyAsync(r->d + r->state_bytes, h, r->prog_bytes, hipMemcpyHostToDevice, L.stream);
         if (e == hipSuccess)
             e = launch_set_basis(s->cfg.dtype, r->d, uint64_t(1) << r->qubits, r->basis, L.stream);
         it = s->pending.erase(it);
@@ -329,10 +350,9 @@ bool admit(hgm_server *s) {
         ++s->active;
         r->t_admit = now_ms(s->t0);
         if (e != hipSuccess) {
-            // the copies may still reference the block: let the stream drain first
+            // the copy may still reference the blocks: let the stream drain first
             hipStreamSynchronize(L.stream);
-            s->arena.release(r->off, r->total_bytes);
-            --s->active;
+            release_req(s, r);
             fail(s, r, HGM_E_HIP, hip_msg(e, "request upload"));
             continue;
         }
@@ -434,8 +454,7 @@ bool launch_step(hgm_server *s, Lane &L) {
             if (s->fatal.empty()) s->fatal = msg;
         }
         for (Req *r : L.active) {
-            s->arena.release(r->off, r->total_bytes);
-            --s->active;
+            release_req(s, r);
             fail(s, r, HGM_E_HIP, msg);
         }
         L.active.clear();
@@ -459,8 +478,7 @@ bool complete_steps(hgm_server *s, Lane &L) {
         const double t = now_ms(s->t0);
         for (size_t k = 0; k < st.finishing.size(); ++k) {
             Req *r = st.finishing[k];
-            s->arena.release(r->off, r->total_bytes);
-            --s->active;
+            release_req(s, r);
             if (q != hipSuccess) {
                 fail(s, r, HGM_E_HIP, hip_msg(q, "serving step"));
                 continue;
@@ -565,7 +583,11 @@ int hgm_server_create(const hgm_server_config *cfg, hgm_server **out) {
         }
     }
     std::string err;
-    if ((rc = s->arena.init(cfg->hbm_budget_bytes, &err))) { set_error(err); return rc; }
+    if ((rc = s->arena.init(cfg->hbm_budget_bytes, false, &err))) { set_error(err); return rc; }
+    // pinned staging for programs: 1/16 of the device budget, at least 64 MiB and at most 1 GiB
+    const size_t stage = std::min<size_t>(size_t(1) << 30,
+                                          std::max<size_t>(size_t(64) << 20, cfg->hbm_budget_bytes / 16));
+    if ((rc = s->staging.init(stage, true, &err))) { s->arena.destroy(); set_error(err); return rc; }
     const size_t m = (size_t)cfg->max_active;
     s->tasks_off = align_up(m * sizeof(BatchItem));
     s->results_off = s->tasks_off + align_up(m * sizeof(ReadoutTask));
@@ -580,6 +602,7 @@ int hgm_server_create(const hgm_server_config *cfg, hgm_server **out) {
             set_error(hip_msg(e, "serving lane set-up"));
             release_lanes(s.get());
             s->arena.destroy();
+            s->staging.destroy();
             return HGM_E_HIP;
         }
     }
@@ -661,6 +684,7 @@ void hgm_server_destroy(hgm_server *s) {
     s->worker.join();
     release_lanes(s);
     s->arena.destroy();
+    s->staging.destroy();
     delete s;
 }
 

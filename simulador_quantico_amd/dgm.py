@@ -256,6 +256,13 @@ class State:
         check(lib.hgm_state_argmax(self._h, C.byref(i), C.byref(p)))
         return i.value, p.value
 
+    def compare(self, other: "State"):
+        """(fidelity, max |a - b|) against another state, computed on the device."""
+        f = C.c_double()
+        m = C.c_double()
+        check(lib.hgm_state_compare(self._h, other._h, C.byref(f), C.byref(m)))
+        return f.value, m.value
+
 
 class Plan:
     """A pass plan (hgm_plan) built on the host; inspectable without a GPU."""
