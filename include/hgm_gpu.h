@@ -56,7 +56,12 @@ enum { HGM_C128 = 0, /* complex double, 16 B/amplitude (default) */
 
 /* pass planners */
 enum { HGM_PLAN_GREEDY = 0, /* dependency-aware lookahead, any set of tile-local bits */
-       HGM_PLAN_WINDOW = 1  /* in-order, contiguous window (the contract of src/core/kernel.cu:199-235) */ };
+       HGM_PLAN_WINDOW = 1, /* in-order, contiguous window (the contract of src/core/kernel.cu:199-235) */
+       /* flag, or-ed with a planner: rewrite each uncontrolled dense gate U as diag(p0, p1) * M with
+        * M's first column real, carrying the diagonal factor forward on its qubit into the next gate
+        * there (diagonal gates and controls commute with it). Same unitary; results differ from the
+        * unfolded list by rounding only (the dense update drops from 16 to 12 multiply-adds per pair). */
+       HGM_PLAN_FOLD_PHASES = 16 };
 
 /* kernel families */
 enum { HGM_KERNEL_AUTO = 0,  /* the register kernel when the tile shape allows it, else LDS */
@@ -196,6 +201,10 @@ int hgm_plan_jit_source(const hgm_plan *p, char *out, size_t cap, size_t *len);
 /* hipRTC compile time of an HGM_KERNEL_JIT plan in seconds (0 when the code
  * came from the process cache or the plan is not JIT). */
 double hgm_plan_compile_seconds(const hgm_plan *p);
+
+/* The gate list after phase folding (HGM_PLAN_FOLD_PHASES), for inspection
+ * and checking: out may be NULL to query *n_out (at most 2 * n_ops + qubits). */
+int hgm_fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out);
 
 /* Tuning check with the reference's rules plus the gfx950 limits; returns
  * HGM_E_TUNING with a message when the tuple is rejected. */

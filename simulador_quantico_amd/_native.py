@@ -17,6 +17,7 @@ HGM_C128 = 0
 HGM_C64 = 1
 HGM_PLAN_GREEDY = 0
 HGM_PLAN_WINDOW = 1
+HGM_PLAN_FOLD_PHASES = 16
 HGM_KERNEL_AUTO = 0
 HGM_KERNEL_LDS = 1
 HGM_KERNEL_REG = 2
@@ -117,6 +118,8 @@ SIGNATURES = [
     ("hgm_plan_jit_source", C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     ("hgm_plan_compile_seconds", C.c_double, [_P]),
     ("hgm_validate_tuning", C.c_int, [C.c_int, C.c_int, C.POINTER(Tuning)]),
+    ("hgm_fold_phases", C.c_int, [C.c_int, C.POINTER(Op), C.c_int64, C.POINTER(Op), C.c_int64,
+                                  C.POINTER(C.c_int64)]),
     ("hgm_dgm_create", C.c_int, [C.c_int, C.POINTER(_P)]),
     ("hgm_dgm_destroy", None, [_P]),
     ("hgm_dgm_set", C.c_int, [_P, C.c_char_p, C.c_int64]),

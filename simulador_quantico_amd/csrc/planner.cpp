@@ -1,6 +1,8 @@
 // Pass planner (see planner.hpp).
 #include "planner.hpp"
 
+#include <complex>
+
 #include <chrono>
 #include <cstring>
 
@@ -112,7 +114,10 @@ int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::
     if (B > 1024) { *err = "Erro de tuning: block_size must be <= 1024 on gfx950"; return HGM_E_TUNING; }
     if (!is_pow2(B) || !is_pow2(R)) { *err = "Erro de tuning: block_size and repeat_count must be powers of two"; return HGM_E_TUNING; }
     if (t.ops_block < 1) { *err = "Erro de tuning: ops_block must be >= 1"; return HGM_E_TUNING; }
-    if (t.planner != HGM_PLAN_GREEDY && t.planner != HGM_PLAN_WINDOW) { *err = "unknown planner"; return HGM_E_TUNING; }
+    if ((t.planner & ~HGM_PLAN_FOLD_PHASES) != HGM_PLAN_GREEDY && (t.planner & ~HGM_PLAN_FOLD_PHASES) != HGM_PLAN_WINDOW) {
+        *err = "unknown planner";
+        return HGM_E_TUNING;
+    }
     if (t.kernel < HGM_KERNEL_AUTO || t.kernel > HGM_KERNEL_JIT) { *err = "unknown kernel family"; return HGM_E_TUNING; }
     if ((size_t(1) << r) * amp_bytes(dtype) > kMaxLdsBytes - kLdsReserve) {
         *err = "Erro de tuning: a tile of 2^" + std::to_string(r) + " amplitudes exceeds the 160 KiB LDS of a gfx950 workgroup";
@@ -141,6 +146,61 @@ int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::
     return HGM_OK;
 }
 
+void fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op> *out) {
+    using C = std::complex<double>;
+    std::vector<C> d0((size_t)qubits, C(1, 0)), d1((size_t)qubits, C(1, 0));
+    auto get = [](const hgm_op &o, int k) { return C(o.m[2 * k], o.m[2 * k + 1]); };
+    auto set = [](hgm_op &o, int k, C v) { o.m[2 * k] = v.real(); o.m[2 * k + 1] = v.imag(); };
+    auto pending = [&](int b) { return d0[b] != C(1, 0) || d1[b] != C(1, 0); };
+    auto flush = [&](int b) {
+        if (!pending(b)) return;
+        hgm_op g{};
+        g.target_bit = b;
+        set(g, 0, d0[b]);
+        set(g, 3, d1[b]);
+        out->push_back(g);
+        d0[b] = d1[b] = C(1, 0);
+    };
+    out->clear();
+    out->reserve((size_t)n_ops + (size_t)qubits);
+    for (int64_t i = 0; i < n_ops; ++i) {
+        hgm_op o = ops[i];
+        const int b = o.target_bit;
+        const int kind = classify(o.m);
+        if (kind == K_DIAG) {            // commutes with every pending diagonal factor
+            out->push_back(o);
+            continue;
+        }
+        if (o.control_mask) {            // the factor on b must act on both control halves first
+            flush(b);
+            out->push_back(o);
+            continue;
+        }
+        // U' = U * diag(d0, d1)
+        const C u00 = get(o, 0) * d0[b], u01 = get(o, 1) * d1[b], u10 = get(o, 2) * d0[b], u11 = get(o, 3) * d1[b];
+        if (kind == K_ANTI) {            // stays anti-diagonal: absorb and emit
+            set(o, 1, u01);
+            set(o, 2, u10);
+            d0[b] = d1[b] = C(1, 0);
+            out->push_back(o);
+            continue;
+        }
+        // U' = diag(p0, p1) * M with the first column of M real and non-negative
+        const double a00 = std::abs(u00), a10 = std::abs(u10);
+        const C p0 = a00 > 0 ? u00 / a00 : C(1, 0), p1 = a10 > 0 ? u10 / a10 : C(1, 0);
+        o.m[0] = a00;
+        o.m[1] = 0.0;
+        set(o, 1, std::conj(p0) * u01);
+        o.m[4] = a10;
+        o.m[5] = 0.0;
+        set(o, 3, std::conj(p1) * u11);
+        d0[b] = p0;
+        d1[b] = p1;
+        out->push_back(o);
+    }
+    for (int b = 0; b < qubits; ++b) flush(b);
+}
+
 int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, const hgm_tuning &t,
                Plan *out, std::string *err) {
     auto t0 = std::chrono::steady_clock::now();
@@ -149,6 +209,18 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
     plan.dtype = dtype;
     int st = make_shape(qubits, dtype, t, &plan.shape, err);
     if (st) return st;
+    std::vector<hgm_op> folded;
+    if ((t.planner & HGM_PLAN_FOLD_PHASES) && n_ops > 0 && ops) {
+        for (int64_t i = 0; i < n_ops; ++i)
+            if (ops[i].target_bit < 0 || ops[i].target_bit >= qubits) {
+                *err = "gate " + std::to_string(i) + ": target bit " + std::to_string(ops[i].target_bit) +
+                       " outside the state";
+                return HGM_E_ARG;
+            }
+        fold_phases(qubits, ops, n_ops, &folded);
+        ops = folded.data();
+        n_ops = (int64_t)folded.size();
+    }
     if (iterations < 0) { *err = "iterations must be >= 0"; return HGM_E_ARG; }
     if (n_ops < 0 || (n_ops > 0 && !ops)) { *err = "bad op array"; return HGM_E_ARG; }
     plan.kernel = t.kernel == HGM_KERNEL_AUTO ? HGM_KERNEL_LDS : t.kernel;
@@ -192,7 +264,7 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
             fill_pass_ops(info, ops, n_ops, chosen, p, plan);
             plan.passes.push_back(p);
         }
-    } else if (t.planner == HGM_PLAN_WINDOW) {
+    } else if ((t.planner & 0xF) == HGM_PLAN_WINDOW) {
         // In order; targets >= c must fit one window of e = r-c contiguous bits.
         const int e = r - c;
         int64_t k = 0;

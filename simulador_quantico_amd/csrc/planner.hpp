@@ -152,7 +152,17 @@ size_t amp_bytes(int dtype);
 // `qubits` qubits. Returns an HGM_* status and fills *err on failure.
 int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::string *err);
 
-// Builds the pass list for ops[0..n_ops) repeated `iterations` times.
+// Equivalent gate list whose uncontrolled dense gates have a real,
+// non-negative first column: each such gate U becomes diag(p0, p1) * M, and
+// the diagonal factor is carried forward on its qubit (diagonal gates and
+// controls commute with it) into the next uncontrolled gate on that qubit, or
+// emitted as a diagonal gate before a controlled gate on it and at the end.
+// A dense update then costs 12 real multiply-adds per amplitude pair instead
+// of 16 (14 when m00 is already real).
+void fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op> *out);
+
+// Builds the pass list for ops[0..n_ops) repeated `iterations` times
+// (after fold_phases when tuning.planner has HGM_PLAN_FOLD_PHASES).
 int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, const hgm_tuning &t,
                Plan *out, std::string *err);
 

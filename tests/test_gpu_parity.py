@@ -184,7 +184,9 @@ def test_random_layers_20q_vs_oracle():
     t, cm, cv, mats = hq.ops_to_arrays(ops)
     want = oracle_run_gates(n, t, cm, cv, mats, basis=0, engine=po.T_PAR_CPU)
     for tun in (N.default_tuning(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4),
-                N.default_tuning(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=2)):
+                N.default_tuning(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=2),
+                N.default_tuning(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
+                                 kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FOLD_PHASES)):
         s = hq.State(n)
         s.set_basis(0)
         st = s.apply(ops, tun)
@@ -299,15 +301,16 @@ def test_jit_kernel_parity(dtype, tol):
     psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
     psi0 /= np.linalg.norm(psi0)
     want = oracle_run_gates(n, t, cm, cv, mats, state=psi0)
-    for B, R, r, c in [(64, 8, 10, 3), (32, 4, 8, 2)]:
+    for B, R, r, c, planner in [(64, 8, 10, 3, N.HGM_PLAN_GREEDY), (32, 4, 8, 2, N.HGM_PLAN_GREEDY),
+                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FOLD_PHASES)]:
         tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c,
-                               kernel=N.HGM_KERNEL_JIT, ops_block=60)
+                               kernel=N.HGM_KERNEL_JIT, ops_block=60, planner=planner)
         plan = hq.Plan(n, ops, tun, 1, dtype)
         assert plan.kernel == N.HGM_KERNEL_JIT
         s = hq.State(n, dtype)
         s.upload(psi0)
         plan.run(s)
-        say("jit", (B, R, r, c), dtype, "passes", len(plan))
+        say("jit", (B, R, r, c, planner), dtype, "passes", len(plan))
         assert np.abs(s.download() - want).max() < tol
 
 
@@ -316,8 +319,8 @@ def test_jit_random_layers_30q_round_trip():
     d = hq.DGM()
     d.setFunction(hq.RandomLayers(n, 20, 1, d))
     ops = d.ops()
-    tun = N.default_tuning(block_size=128, repeat_count=8, gpu_region_bits=11, gpu_coalesced_bits=3,
-                           kernel=N.HGM_KERNEL_JIT)
+    tun = N.default_tuning(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
+                           kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FOLD_PHASES)
     s = hq.State(n)
     s.set_basis(0)
     st = hq.Plan(n, ops, tun).run(s)
