@@ -56,21 +56,25 @@ def algorithmic_flops(ops, n: int, iterations: int) -> float:
 METRIC = "amp-updates/s + %HBM roofline @30q; circuits/s and p99 latency on serving mix"
 
 WORKLOADS = {
-    # name: (qubits, dtype, builder, tuning)
+    # name: (qubits, dtype, builder, tuning); planner 48 = greedy + gate fusion + phase folding
     "random30": dict(qubits=30, dtype="c128", kind="random", depth=20, seed=1,
                      tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
-                                 kernel=3)),
+                                 kernel=3, planner=48)),
     "u3layers30": dict(qubits=30, dtype="c128", kind="u3layers", depth=20, seed=1,
-                       tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=2)),
+                       tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
+                                   kernel=3, planner=48)),
     "random32f": dict(qubits=32, dtype="c64", kind="random", depth=20, seed=1,
                       tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
-                                  kernel=3)),
+                                  kernel=3, planner=48)),
     "hadamard20": dict(qubits=20, dtype="c128", kind="hadamard", layers=1,
-                       tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4)),
+                       tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4,
+                                   planner=48)),
     "grover22": dict(qubits=22, dtype="c128", kind="grover", seed=22,
-                     tuning=dict(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=4)),
+                     tuning=dict(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=4,
+                                 kernel=2, planner=48)),
     "draper29": dict(qubits=29, dtype="c128", kind="draper", seed=29,
-                     tuning=dict(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=4)),
+                     tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
+                                 kernel=3, planner=48)),
 }
 
 
@@ -181,9 +185,9 @@ def fidelity_report(names=("hadamard20", "grover22", "draper29", "random30", "ra
         row = {"circuit": title}
         states = {}
         for dt, tun in ((np.complex128, dict(block_size=256, repeat_count=4, gpu_region_bits=11,
-                                               gpu_coalesced_bits=2, kernel=2)),
+                                               gpu_coalesced_bits=2, kernel=2, planner=48)),
                         (np.complex64, dict(block_size=256, repeat_count=8, gpu_region_bits=12,
-                                              gpu_coalesced_bits=3, kernel=2))):
+                                              gpu_coalesced_bits=3, kernel=2, planner=48))):
             s = hq.State(n, dt)
             s.set_basis(basis)
             st = s.apply(ops, N.default_tuning(**tun), iterations=iterations)

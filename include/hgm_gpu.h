@@ -61,7 +61,12 @@ enum { HGM_PLAN_GREEDY = 0, /* dependency-aware lookahead, any set of tile-local
         * M's first column real, carrying the diagonal factor forward on its qubit into the next gate
         * there (diagonal gates and controls commute with it). Same unitary; results differ from the
         * unfolded list by rounding only (the dense update drops from 16 to 12 multiply-adds per pair). */
-       HGM_PLAN_FOLD_PHASES = 16 };
+       HGM_PLAN_FOLD_PHASES = 16,
+       /* flag: fuse every run of uncontrolled gates on one qubit into one gate (their product),
+        * placed before the first later gate on that qubit it does not commute with; products equal
+        * to the identity vanish. Iterations are expanded first, so runs fuse across them. Same
+        * unitary; results differ by rounding only. */
+       HGM_PLAN_FUSE = 32 };
 
 /* kernel families */
 enum { HGM_KERNEL_AUTO = 0,  /* the register kernel when the tile shape allows it, else LDS */
@@ -205,6 +210,8 @@ double hgm_plan_compile_seconds(const hgm_plan *p);
 /* The gate list after phase folding (HGM_PLAN_FOLD_PHASES), for inspection
  * and checking: out may be NULL to query *n_out (at most 2 * n_ops + qubits). */
 int hgm_fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out);
+/* The gate list after fusion (HGM_PLAN_FUSE), same conventions (at most n_ops + qubits gates). */
+int hgm_fuse_gates(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out);
 
 /* Tuning check with the reference's rules plus the gfx950 limits; returns
  * HGM_E_TUNING with a message when the tuple is rejected. */

@@ -177,6 +177,28 @@ __device__ __forceinline__ void reg_dense(typename Vec<T>::type (&a)[A], const M
     }
 }
 
+// Real 2x2 matrix (imaginary parts zero): half the multiply-adds of reg_dense.
+template <class T, int A, int BIT, bool COND>
+__device__ __forceinline__ void reg_dense_real(typename Vec<T>::type (&a)[A], const Mat<T> &m, uint32_t rmask,
+                                               uint32_t rval) {
+    using V = typename Vec<T>::type;
+#pragma unroll
+    for (int k0 = 0; k0 < A; ++k0) {
+        if (k0 & (1 << BIT)) continue;
+        const int k1 = k0 | (1 << BIT);
+        const bool id = COND && (((uint32_t)k0 & rmask) != rval);
+        const T u0 = id ? T(1) : m.m0r, u1 = id ? T(0) : m.m1r, u2 = id ? T(0) : m.m2r, u3 = id ? T(1) : m.m3r;
+        const V x = a[k0], y = a[k1];
+        V p, q;
+        p.x = u0 * x.x + u1 * y.x;
+        p.y = u0 * x.y + u1 * y.y;
+        q.x = u2 * x.x + u3 * y.x;
+        q.y = u2 * x.y + u3 * y.y;
+        a[k0] = p;
+        a[k1] = q;
+    }
+}
+
 template <class T, int A, int BIT>
 __device__ __forceinline__ void reg_anti(typename Vec<T>::type (&a)[A], const Mat<T> &m) {
     using V = typename Vec<T>::type;
@@ -228,6 +250,19 @@ __device__ __forceinline__ void reg_gate(typename Vec<T>::type (&a)[A], const Re
 #define HGM_CALL(B) reg_anti<T, A, B>(a, m)
         HGM_REG_BITS(HGM_CALL)
 #undef HGM_CALL
+        return;
+    }
+    if (o.kind == R_DENSE_REAL) {
+        if (o.tmask) m = or_identity(m, ((uint32_t)t & o.tmask) != o.tval);   // per lane, once per gate
+        if (o.rmask) {
+#define HGM_CALL(B) reg_dense_real<T, A, B, true>(a, m, o.rmask, o.rval)
+            HGM_REG_BITS(HGM_CALL)
+#undef HGM_CALL
+        } else {
+#define HGM_CALL(B) reg_dense_real<T, A, B, false>(a, m, 0u, 0u)
+            HGM_REG_BITS(HGM_CALL)
+#undef HGM_CALL
+        }
         return;
     }
     if (o.tmask == 0) {   // tile-uniform matrix (scalar registers)

@@ -114,7 +114,8 @@ int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::
     if (B > 1024) { *err = "Erro de tuning: block_size must be <= 1024 on gfx950"; return HGM_E_TUNING; }
     if (!is_pow2(B) || !is_pow2(R)) { *err = "Erro de tuning: block_size and repeat_count must be powers of two"; return HGM_E_TUNING; }
     if (t.ops_block < 1) { *err = "Erro de tuning: ops_block must be >= 1"; return HGM_E_TUNING; }
-    if ((t.planner & ~HGM_PLAN_FOLD_PHASES) != HGM_PLAN_GREEDY && (t.planner & ~HGM_PLAN_FOLD_PHASES) != HGM_PLAN_WINDOW) {
+    const int base_planner = t.planner & ~(HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE);
+    if (base_planner != HGM_PLAN_GREEDY && base_planner != HGM_PLAN_WINDOW) {
         *err = "unknown planner";
         return HGM_E_TUNING;
     }
@@ -144,6 +145,53 @@ int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::
     }
     *out = s;
     return HGM_OK;
+}
+
+void fuse_gates(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op> *out) {
+    using C = std::complex<double>;
+    struct M2 { C a, b, c, d; };   // row-major 2x2
+    const M2 I{C(1, 0), C(0, 0), C(0, 0), C(1, 0)};
+    std::vector<M2> P((size_t)qubits, I);
+    std::vector<char> has((size_t)qubits, 0);
+    auto mat = [](const hgm_op &o) {
+        return M2{C(o.m[0], o.m[1]), C(o.m[2], o.m[3]), C(o.m[4], o.m[5]), C(o.m[6], o.m[7])};
+    };
+    auto mul = [](const M2 &x, const M2 &y) {   // x * y
+        return M2{x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d, x.c * y.a + x.d * y.c, x.c * y.b + x.d * y.d};
+    };
+    auto diag = [](const M2 &m) { return m.b == C(0, 0) && m.c == C(0, 0); };
+    auto emit = [&](int q) {
+        if (!has[q]) return;
+        has[q] = 0;
+        const M2 m = P[q];
+        P[q] = I;
+        if (m.a == C(1, 0) && m.b == C(0, 0) && m.c == C(0, 0) && m.d == C(1, 0)) return;   // cancelled
+        hgm_op g{};
+        g.target_bit = q;
+        const C v[4] = {m.a, m.b, m.c, m.d};
+        for (int k = 0; k < 4; ++k) { g.m[2 * k] = v[k].real(); g.m[2 * k + 1] = v[k].imag(); }
+        out->push_back(g);
+    };
+    out->clear();
+    out->reserve((size_t)n_ops + (size_t)qubits);
+    for (int64_t i = 0; i < n_ops; ++i) {
+        const hgm_op &o = ops[i];
+        const int q = o.target_bit;
+        if (!o.control_mask) {            // accumulate: the gate acts after what is pending on q
+            P[q] = has[q] ? mul(mat(o), P[q]) : mat(o);
+            has[q] = 1;
+            continue;
+        }
+        // a controlled gate: pending factors it does not commute with go first
+        const bool d = classify(o.m) == K_DIAG;
+        if (has[q] && !(d && diag(P[q]))) emit(q);
+        for (uint64_t cm = o.control_mask; cm; cm &= cm - 1) {
+            const int c = __builtin_ctzll(cm);
+            if (has[c] && !diag(P[c])) emit(c);
+        }
+        out->push_back(o);
+    }
+    for (int q = 0; q < qubits; ++q) emit(q);
 }
 
 void fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op> *out) {
@@ -209,17 +257,38 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
     plan.dtype = dtype;
     int st = make_shape(qubits, dtype, t, &plan.shape, err);
     if (st) return st;
-    std::vector<hgm_op> folded;
-    if ((t.planner & HGM_PLAN_FOLD_PHASES) && n_ops > 0 && ops) {
+    std::vector<hgm_op> expanded, fused, folded;
+    if ((t.planner & (HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE)) && n_ops > 0 && ops) {
         for (int64_t i = 0; i < n_ops; ++i)
-            if (ops[i].target_bit < 0 || ops[i].target_bit >= qubits) {
+            if (ops[i].target_bit < 0 || ops[i].target_bit >= qubits || (ops[i].control_mask >> ops[i].target_bit) & 1) {
                 *err = "gate " + std::to_string(i) + ": target bit " + std::to_string(ops[i].target_bit) +
-                       " outside the state";
+                       " outside the state or among its controls";
                 return HGM_E_ARG;
             }
-        fold_phases(qubits, ops, n_ops, &folded);
-        ops = folded.data();
-        n_ops = (int64_t)folded.size();
+        if (t.planner & HGM_PLAN_FUSE) {
+            if (iterations > 1) {   // fuse across iteration boundaries too
+                expanded.reserve((size_t)(n_ops * iterations));
+                for (int it = 0; it < iterations; ++it) expanded.insert(expanded.end(), ops, ops + n_ops);
+                ops = expanded.data();
+                n_ops = (int64_t)expanded.size();
+                iterations = 1;
+            }
+            fuse_gates(qubits, ops, n_ops, &fused);
+            ops = fused.data();
+            n_ops = (int64_t)fused.size();
+        }
+        if (t.planner & HGM_PLAN_FOLD_PHASES) {
+            fold_phases(qubits, ops, n_ops, &folded);
+            ops = folded.data();
+            n_ops = (int64_t)folded.size();
+        }
+        if (n_ops == 0) {   // the whole list cancelled: one identity keeps the plan well formed
+            hgm_op g{};
+            g.m[0] = g.m[6] = 1.0;
+            folded.assign(1, g);
+            ops = folded.data();
+            n_ops = 1;
+        }
     }
     if (iterations < 0) { *err = "iterations must be >= 0"; return HGM_E_ARG; }
     if (n_ops < 0 || (n_ops > 0 && !ops)) { *err = "bad op array"; return HGM_E_ARG; }
@@ -532,6 +601,8 @@ void build_register_program(Plan &plan) {
                     // conditioned anti-diagonal gates run as dense ones: the
                     // kernel resolves conditions by selecting matrix or identity
                     if (o.kind != R_DENSE && (o.rmask || o.tmask)) o.kind = R_DENSE;
+                    if (o.kind == R_DENSE && d.m[1] == 0.0 && d.m[3] == 0.0 && d.m[5] == 0.0 && d.m[7] == 0.0)
+                        o.kind = R_DENSE_REAL;
                     o.gmask = d.gmask;
                     o.gval = d.gval;
                     std::memcpy(o.m, d.m, sizeof(o.m));

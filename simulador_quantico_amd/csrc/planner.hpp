@@ -87,7 +87,8 @@ enum RegKind : int32_t {
     R_PHASE = 2,       // multiply by (m[0], m[1]) where all three conditions hold
     R_REMAP = 3,       // move to phase map `rbit` through LDS
     R_SWAP = 4,        // exchange of the pair (the X matrix)
-    R_PHASE_REAL = 5   // R_PHASE with a real factor m[0] (m[1] == 0)
+    R_PHASE_REAL = 5,  // R_PHASE with a real factor m[0] (m[1] == 0)
+    R_DENSE_REAL = 6   // R_DENSE with a real matrix (imaginary parts exactly 0)
 };
 
 struct RegOp {
@@ -161,8 +162,16 @@ int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::
 // of 16 (14 when m00 is already real).
 void fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op> *out);
 
+// Equivalent gate list in which every run of uncontrolled gates on one qubit
+// (with nothing in between that touches the qubit, except diagonal gates
+// while the run so far is diagonal) is one gate: the product of the run,
+// placed where the run's first non-commuting successor needs it, or at the
+// end; products equal to the identity are dropped.
+void fuse_gates(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op> *out);
+
 // Builds the pass list for ops[0..n_ops) repeated `iterations` times
-// (after fold_phases when tuning.planner has HGM_PLAN_FOLD_PHASES).
+// (after fuse_gates and fold_phases when tuning.planner has HGM_PLAN_FUSE /
+// HGM_PLAN_FOLD_PHASES; fusing first expands the iterations).
 int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, const hgm_tuning &t,
                Plan *out, std::string *err);
 

@@ -542,21 +542,35 @@ int hgm_plan_build(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int 
 
 void hgm_plan_destroy(hgm_plan *p) { delete p; }
 
-int hgm_fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out) {
+namespace {
+int rewrite_ops(bool fuse, int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out) {
     if (!n_out || n_ops < 0 || (n_ops > 0 && !ops) || qubits < 1 || qubits > kMaxQubits) {
         t_err = "bad argument";
         return HGM_E_ARG;
     }
     for (int64_t i = 0; i < n_ops; ++i)
-        if (ops[i].target_bit < 0 || ops[i].target_bit >= qubits) { t_err = "target bit outside the state"; return HGM_E_ARG; }
+        if (ops[i].target_bit < 0 || ops[i].target_bit >= qubits || (ops[i].control_mask >> ops[i].target_bit) & 1) {
+            t_err = "target bit outside the state or among the controls";
+            return HGM_E_ARG;
+        }
     std::vector<hgm_op> v;
-    fold_phases(qubits, ops, n_ops, &v);
+    if (fuse) fuse_gates(qubits, ops, n_ops, &v);
+    else fold_phases(qubits, ops, n_ops, &v);
     *n_out = (int64_t)v.size();
     if (out) {
         if (cap < (int64_t)v.size()) { t_err = "output buffer too small"; return HGM_E_ARG; }
         std::memcpy(out, v.data(), v.size() * sizeof(hgm_op));
     }
     return HGM_OK;
+}
+}  // namespace
+
+int hgm_fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out) {
+    return rewrite_ops(false, qubits, ops, n_ops, out, cap, n_out);
+}
+
+int hgm_fuse_gates(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out) {
+    return rewrite_ops(true, qubits, ops, n_ops, out, cap, n_out);
 }
 
 int64_t hgm_plan_num_passes(const hgm_plan *p) { return p ? (int64_t)p->plan.passes.size() : -1; }

@@ -147,6 +147,10 @@ double now_ms(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Gate lists are fused and phase-folded before planning (same unitary, fewer
+// and real-valued gates: Grover's X-H-X runs become one real gate per qubit).
+constexpr int kServingPlanner = HGM_PLAN_GREEDY | HGM_PLAN_FUSE | HGM_PLAN_FOLD_PHASES;
+
 // Tile shape the serving engine uses for a qubit count when no autotuning is
 // asked for: 16 amplitudes per thread; states up to 13 qubits are one tile;
 // larger ones use tiles of 2^12 amplitudes with 3 coalesced bits.
@@ -155,6 +159,7 @@ hgm_tuning serving_default_tuning(int n) {
     hgm_default_tuning(&t);
     t.ops_block = 1 << 30;
     t.kernel = HGM_KERNEL_REG;
+    t.planner = kServingPlanner;
     if (n <= 13) {
         const int64_t pairs = int64_t(1) << (n - 1);
         t.repeat_count = (int)std::min<int64_t>(8, pairs);
@@ -579,6 +584,7 @@ int hgm_server_create(const hgm_server_config *cfg, hgm_server **out) {
             if ((rc = hgm_autotune(n, cfg->dtype, ops.data(), (int64_t)ops.size(), 1, HGM_KERNEL_REG, &best, &ms)))
                 return rc;
             best.ops_block = 1 << 30;
+            best.planner = kServingPlanner;
             s->tune[n] = best;
         }
     }

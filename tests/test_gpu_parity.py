@@ -86,7 +86,8 @@ def test_random_gates_tuning_sweep(n):
     psi0 /= np.linalg.norm(psi0)
     want = oracle_run_gates(n, t, cm, cv, mats, state=psi0)
     for B, R, r, c in TUNINGS:
-        for planner in (N.HGM_PLAN_GREEDY, N.HGM_PLAN_WINDOW):
+        for planner in (N.HGM_PLAN_GREEDY, N.HGM_PLAN_WINDOW,
+                        N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_FOLD_PHASES):
             for ops_block in (300, 5):
                 say("sweep", n, (B, R, r, c), planner, ops_block)
                 tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c,
@@ -135,17 +136,19 @@ def test_grover_22q_config():
     ops = d.ops()
     d.setFunction(hq.Hadamard(n, 0, n))
     hops = d.ops()
-    tun = N.default_tuning(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=4)
-    s = hq.State(n)
-    s.set_basis(0)
-    s.apply(hops, tun)
-    st = s.apply(ops, tun, iterations=k)
-    say("grover22 passes", st["passes"], "kernel_ms", st["kernel_ms"])
-    idx, p = s.argmax()
-    assert idx == marked
-    psi = s.download()
-    assert np.abs(psi - grover_closed_form(n, marked, k)).max() < 1e-10
-    assert abs(p - np.sin((2 * k + 1) * np.arcsin(2 ** (-n / 2))) ** 2) < 1e-10
+    for planner in (N.HGM_PLAN_GREEDY, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_FOLD_PHASES):
+        tun = N.default_tuning(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=4,
+                               planner=planner)
+        s = hq.State(n)
+        s.set_basis(0)
+        s.apply(hops, tun)
+        st = s.apply(ops, tun, iterations=k)
+        say("grover22 planner", planner, "passes", st["passes"], "kernel_ms", st["kernel_ms"])
+        idx, p = s.argmax()
+        assert idx == marked
+        psi = s.download()
+        assert np.abs(psi - grover_closed_form(n, marked, k)).max() < 1e-10
+        assert abs(p - np.sin((2 * k + 1) * np.arcsin(2 ** (-n / 2))) ** 2) < 1e-10
 
 
 def test_draper_adder_29q_config():
@@ -302,7 +305,8 @@ def test_jit_kernel_parity(dtype, tol):
     psi0 /= np.linalg.norm(psi0)
     want = oracle_run_gates(n, t, cm, cv, mats, state=psi0)
     for B, R, r, c, planner in [(64, 8, 10, 3, N.HGM_PLAN_GREEDY), (32, 4, 8, 2, N.HGM_PLAN_GREEDY),
-                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FOLD_PHASES)]:
+                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FOLD_PHASES),
+                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_FOLD_PHASES)]:
         tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c,
                                kernel=N.HGM_KERNEL_JIT, ops_block=60, planner=planner)
         plan = hq.Plan(n, ops, tun, 1, dtype)

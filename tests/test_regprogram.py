@@ -13,7 +13,7 @@ import simulador_quantico_amd as hq
 from simulador_quantico_amd import _native as N
 from tests.test_oracle import _random_gate_list, oracle_run_gates
 
-R_DENSE, R_ANTI, R_PHASE, R_REMAP = 0, 1, 2, 3
+R_DENSE, R_ANTI, R_PHASE, R_REMAP, R_SWAP, R_PHASE_REAL, R_DENSE_REAL = 0, 1, 2, 3, 4, 5, 6
 
 
 def pdep(v, mask):
@@ -70,7 +70,7 @@ def emulate(n, plan, psi):
                     continue
                 lane = (t & int(o["tmask"])) == int(o["tval"])
                 m = o["m"][0::2] + 1j * o["m"][1::2]
-                if kind == R_PHASE:
+                if kind in (R_PHASE, R_PHASE_REAL):
                     for k in range(A):
                         if (k & int(o["rmask"])) == int(o["rval"]):
                             a[lane, k] *= m[0]
@@ -81,7 +81,7 @@ def emulate(n, plan, psi):
                         continue
                     k1 = k0 | (1 << bit)
                     x, y = a[lane, k0].copy(), a[lane, k1].copy()
-                    if kind == R_DENSE:
+                    if kind in (R_DENSE, R_DENSE_REAL):
                         a[lane, k0] = m[0] * x + m[1] * y
                         a[lane, k1] = m[2] * x + m[3] * y
                     else:
