@@ -199,6 +199,39 @@ std::string genRot(int qubits, int reg, long phase_bits, Gates &g);
 std::vector<std::string> CSwapR(int qubits, int ctrl, int reg1, int reg2, int width);
 std::vector<std::string> SwapOver(int qubits, int reg, int width);
 
+// Modular arithmetic on Fourier-basis registers (src/algorithms/lib_shor_circuits.cpp:47-166).
+std::vector<std::string> C2AddMod(int qubits, int ctrl1, int ctrl2, int reg, int over, int over_bool, int width,
+                                  long a, long N, Gates &g);
+std::vector<std::string> C2SubMod(int qubits, int ctrl1, int ctrl2, int reg, int over, int over_bool, int width,
+                                  long a, long N, Gates &g);
+std::vector<std::string> CMultMod(int qubits, int ctrl, int reg1, int reg2, int over, int over_bool, int width,
+                                  long a, long N, Gates &g);
+std::vector<std::string> CRMultMod(int qubits, int ctrl, int reg1, int reg2, int over, int over_bool, int width,
+                                   long a, long N, Gates &g);
+
+// Classical helpers of Shor's algorithm (src/algorithms/lib_shor_number_theory.cpp,
+// src/core/common.cpp:209-219).
+int revert_bits(int value, int bits);
+int quantum_gcd(int a, int b);
+void quantum_frac_approx(int *num, int *den, int width);
+long mul_inv(long value, long modulus);
+long modular_pow(long base, long exponent, long modulus);
+
+struct ShorResult {
+    std::vector<int> factors;   // {f, N/f}, or empty when this attempt failed
+    long base = 0;              // the random base drawn from g_rng
+    long phase_bits = 0;        // measured phase bits, first round in the highest bit
+    int qubits = 0;
+    int64_t gates = 0;          // gates executed over all rounds
+    double kernel_ms = 0;       // pass-kernel time over all rounds
+};
+
+// One attempt of Shor's algorithm with semiclassical phase estimation
+// (src/algorithms/lib_shor.cpp:17-163): 2*width+3 qubits, 2*width rounds on
+// one executor configured from cfg (exec_type, tuning, resident, ...).
+template <class Real>
+ShorResult Shor(long N, const DGMCore &cfg);
+
 // The step list Grover() runs after its Hadamard layer, one iteration
 // (ancilla on qubit 0; src/algorithms/lib_grover.cpp:34-51).
 std::vector<std::string> GroverIterationRef(long qubits, long search_value);

@@ -131,7 +131,9 @@ const char *hgm_version(void);
 int hgm_init(int device);
 /* Number of visible HIP devices (0 when none). */
 int hgm_device_count(int *count);
-/* Free cached device buffers held for hgm_execute. */
+/* Free the device buffers the library caches between calls (the hgm_execute
+ * state buffer, op tables, register programs) and unload plan-compiled kernel
+ * modules; they are rebuilt on demand. */
 int hgm_release_cache(void);
 
 /* ------------------------------------------------------------------ one-shot
@@ -319,6 +321,20 @@ int hgm_dgm_sync_host(hgm_dgm *d);                             /* resident mode:
 int hgm_dgm_stats(const hgm_dgm *d, hgm_stats *out);
 int hgm_dgm_count_ops(hgm_dgm *d, int iterations, int64_t counts[7]); /* CountOps, dgm_core.cpp:270-299 */
 void hgm_seed_rng(uint64_t seed);                              /* seed_rng, common.cpp:12-14 */
+/* Classical steps of Shor's algorithm: continued-fraction approximation of
+ * num/den with denominator below 2^width (replaces quantum_frac_approx,
+ * src/algorithms/lib_shor_number_theory.cpp:43-71, same mixed float/double
+ * arithmetic), modular inverse (mul_inv, :77-90) and bit reversal
+ * (revert_bits, :14-22). */
+void hgm_frac_approx(int *num, int *den, int width);
+int64_t hgm_mul_inv(int64_t value, int64_t modulus);
+int hgm_revert_bits(int value, int bits);
+/* One attempt of Shor's algorithm (src/algorithms/lib_shor.cpp:17-163) on an
+ * executor configured like cfg (dtype, exec_type, tuning, resident; cfg's own
+ * state is not touched): out = {factor, N / factor, base, measured phase
+ * bits}, factors 0 when the attempt failed (the algorithm is probabilistic).
+ * The base and the measurements draw from the generator seeded by hgm_seed_rng. */
+int hgm_shor(hgm_dgm *cfg, int64_t number, int64_t out[4], hgm_stats *stats);
 
 /* Circuit step generators and builders by name (src/core/gates.cpp:69-163,
  * src/algorithms/lib_grover.cpp, src/algorithms/lib_shor_circuits.cpp, plus

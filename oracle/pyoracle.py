@@ -192,6 +192,11 @@ def _load_ref():
         lib.ref_gen.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_long), C.c_int, C.c_char_p,
                                 C.c_char_p, C.c_size_t]
         lib.ref_grover_iteration.argtypes = [C.c_long, C.c_long, C.c_char_p, C.c_size_t]
+        lib.ref_frac_approx.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]
+        lib.ref_mul_inv.argtypes = [C.c_long, C.c_long]
+        lib.ref_mul_inv.restype = C.c_long
+        lib.ref_revert_bits.argtypes = [C.c_int, C.c_int]
+        lib.ref_shor.argtypes = [C.c_long, C.POINTER(C.c_int)]
         _ref = lib
     return _ref
 
@@ -274,3 +279,26 @@ def ref_grover_iteration(qubits: int, search_value: int) -> list[str]:
     buf = C.create_string_buffer(n + 1)
     lib.ref_grover_iteration(qubits, search_value, buf, n + 1)
     return buf.value.decode().split(";")
+
+
+def ref_frac_approx(num: int, den: int, width: int) -> tuple[int, int]:
+    """The reference's quantum_frac_approx (lib_shor_number_theory.cpp:43-71)."""
+    n, d = C.c_int(num), C.c_int(den)
+    _load_ref().ref_frac_approx(C.byref(n), C.byref(d), width)
+    return n.value, d.value
+
+
+def ref_mul_inv(v: int, m: int) -> int:
+    return int(_load_ref().ref_mul_inv(v, m))
+
+
+def ref_revert_bits(v: int, bits: int) -> int:
+    return int(_load_ref().ref_revert_bits(v, bits))
+
+
+def ref_shor(number: int, seed: int) -> tuple[int, int]:
+    """One attempt of the reference's Shor() on its t_CPU engine after seed_rng(seed)."""
+    ref_seed(seed)
+    out = (C.c_int * 2)()
+    _load_ref().ref_shor(number, out)
+    return out[0], out[1]

@@ -158,7 +158,24 @@ int ref_gen(void *p, const char *which, const long *a, int na, const char *op, c
     if (w == "CAddF") return put(CAddF(a[0], a[1], a[2], a[3], a[4], a[5], d->gates), out, cap);
     if (w == "genRot") return put(genRot(a[0], a[1], a[2], d->gates), out, cap);
     if (w == "CSwapR") return put(join(CSwapR(a[0], a[1], a[2], a[3], a[4])), out, cap);
+    if (w == "C2AddMod") return put(join(C2AddMod(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], d->gates)), out, cap);
+    if (w == "C2SubMod") return put(join(C2SubMod(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], d->gates)), out, cap);
+    if (w == "CMultMod") return put(join(CMultMod(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], d->gates)), out, cap);
+    if (w == "CRMultMod") return put(join(CRMultMod(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], d->gates)), out, cap);
     return -1;
+}
+
+// The reference's classical helpers of Shor's algorithm.
+void ref_frac_approx(int *num, int *den, int width) { quantum_frac_approx(num, den, width); }
+long ref_mul_inv(long v, long m) { return mul_inv(v, m); }
+int ref_revert_bits(int v, int bits) { return revert_bits(v, bits); }
+
+// One attempt of the reference's Shor() with its t_CPU engine; out = {f, N/f}
+// or {0, 0}.
+void ref_shor(long number, int *out) {
+    std::vector<int> f = Shor(number, t_CPU, 1, 14, 11, 1, 8, 4, 64, 2);
+    out[0] = f.size() == 2 ? f[0] : 0;
+    out[1] = f.size() == 2 ? f[1] : 0;
 }
 
 // The step list Grover() (src/algorithms/lib_grover.cpp:15-66) executes after

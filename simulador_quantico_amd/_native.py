@@ -146,6 +146,10 @@ SIGNATURES = [
     ("hgm_dgm_stats", C.c_int, [_P, C.POINTER(Stats)]),
     ("hgm_dgm_count_ops", C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
     ("hgm_seed_rng", None, [C.c_uint64]),
+    ("hgm_shor", C.c_int, [_P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(Stats)]),
+    ("hgm_frac_approx", None, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]),
+    ("hgm_mul_inv", C.c_int64, [C.c_int64, C.c_int64]),
+    ("hgm_revert_bits", C.c_int, [C.c_int, C.c_int]),
     ("hgm_generate", C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64), C.c_int, C.c_char_p, C.c_char_p, C.c_size_t,
                                C.POINTER(C.c_size_t)]),
     ("hgm_autotune", C.c_int, [C.c_int, C.c_int, C.POINTER(Op), C.c_int64, C.c_int, C.c_int, C.POINTER(Tuning),

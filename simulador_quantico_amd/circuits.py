@@ -137,3 +137,31 @@ def RandomLayers(qubits, depth, seed, dgm):
 def DraperAdder(width, dgm):
     """QFT-based adder on |a>_w |carry,b>_(w+1) -> |a, a+b>; registers R/R' rotations in dgm."""
     return _steps(_gen("DraperAdder", [width], dgm=dgm))
+
+
+def C2AddMod(qubits, ctrl1, ctrl2, reg, over, over_bool, width, a, N, dgm):
+    return _steps(_gen("C2AddMod", [qubits, ctrl1, ctrl2, reg, over, over_bool, width, a, N], dgm=dgm))
+
+
+def C2SubMod(qubits, ctrl1, ctrl2, reg, over, over_bool, width, a, N, dgm):
+    return _steps(_gen("C2SubMod", [qubits, ctrl1, ctrl2, reg, over, over_bool, width, a, N], dgm=dgm))
+
+
+def CMultMod(qubits, ctrl, reg1, reg2, over, over_bool, width, a, N, dgm):
+    return _steps(_gen("CMultMod", [qubits, ctrl, reg1, reg2, over, over_bool, width, a, N], dgm=dgm))
+
+
+def CRMultMod(qubits, ctrl, reg1, reg2, over, over_bool, width, a, N, dgm):
+    return _steps(_gen("CRMultMod", [qubits, ctrl, reg1, reg2, over, over_bool, width, a, N], dgm=dgm))
+
+
+def Shor(number, dgm):
+    """One attempt of Shor's algorithm on an executor configured like `dgm`
+    (src/algorithms/lib_shor.cpp:17-163). Returns (factors, base, phase_bits,
+    stats): factors is () when the attempt failed."""
+    from ._native import Stats
+    out = (C.c_int64 * 4)()
+    st = Stats()
+    check(lib.hgm_shor(dgm._h, int(number), out, C.byref(st)))
+    f = (out[0], out[1]) if out[0] else ()
+    return f, out[2], out[3], st.as_dict()

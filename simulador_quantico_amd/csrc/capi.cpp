@@ -243,6 +243,32 @@ int hgm_dgm_count_ops(hgm_dgm *d, int iterations, int64_t counts[7]) {
 
 void hgm_seed_rng(uint64_t seed) { hgm::seed_rng((unsigned long)seed); }
 
+void hgm_frac_approx(int *num, int *den, int width) { hgm::quantum_frac_approx(num, den, width); }
+int64_t hgm_mul_inv(int64_t value, int64_t modulus) { return hgm::mul_inv((long)value, (long)modulus); }
+int hgm_revert_bits(int value, int bits) { return hgm::revert_bits(value, bits); }
+
+int hgm_shor(hgm_dgm *cfg, int64_t number, int64_t out[4], hgm_stats *stats) {
+    if (!cfg || !out) { hgm::set_error("null argument"); return HGM_E_ARG; }
+    if (number < 4 || number > (1 << 14)) {
+        hgm::set_error("Shor needs 4 <= N <= 16384 (2*bits(N)+3 qubits)");
+        return HGM_E_ARG;
+    }
+    return guard([&] {
+        const hgm::DGMCore &c = cfg->core();
+        const hgm::ShorResult r = cfg->d64 ? hgm::Shor<double>((long)number, c) : hgm::Shor<float>((long)number, c);
+        out[0] = r.factors.size() == 2 ? r.factors[0] : 0;
+        out[1] = r.factors.size() == 2 ? r.factors[1] : 0;
+        out[2] = r.base;
+        out[3] = r.phase_bits;
+        if (stats) {
+            std::memset(stats, 0, sizeof *stats);
+            stats->gates = r.gates;
+            stats->kernel_ms = r.kernel_ms;
+        }
+        return HGM_OK;
+    });
+}
+
 int hgm_generate(hgm_dgm *d, const char *which, const int64_t *a, int na, const char *op, char *out, size_t cap,
                  size_t *len) {
     if (!which || !len || (na > 0 && !a)) return HGM_E_ARG;
@@ -288,6 +314,19 @@ int hgm_generate(hgm_dgm *d, const char *which, const int64_t *a, int na, const 
         else if (w == "GroverIterations") { need(1); s = std::to_string(hgm::GroverIterations(a[0])); }
         else if (w == "RandomLayers") { need(3); s = join(hgm::RandomLayers(a[0], a[1], (uint64_t)a[2], gates())); }
         else if (w == "DraperAdder") { need(1); s = join(hgm::DraperAdder(a[0], gates())); }
+        else if (w == "C2AddMod") {
+            need(9);
+            s = join(hgm::C2AddMod(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], gates()));
+        } else if (w == "C2SubMod") {
+            need(9);
+            s = join(hgm::C2SubMod(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], gates()));
+        } else if (w == "CMultMod") {
+            need(9);
+            s = join(hgm::CMultMod(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], gates()));
+        } else if (w == "CRMultMod") {
+            need(9);
+            s = join(hgm::CRMultMod(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], gates()));
+        }
         else throw hgm::Error(HGM_E_ARG, "unknown generator '" + w + "'");
         *len = s.size();
         if (out && cap > s.size()) std::memcpy(out, s.c_str(), s.size() + 1);

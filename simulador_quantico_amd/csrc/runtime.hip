@@ -340,8 +340,20 @@ int hgm_device_count(int *count) {
 int hgm_release_cache(void) {
     Ctx &c = ctx();
     std::lock_guard<std::mutex> g(c.mu);
-    if (c.cache) { hipStreamSynchronize(c.stream); hipFree(c.cache); c.cache = nullptr; c.cache_cap = 0; }
-    if (c.ops) { hipStreamSynchronize(c.stream); hipFree(c.ops); c.ops = nullptr; c.ops_cap = 0; }
+    if (!c.ready) return HGM_OK;
+    hipStreamSynchronize(c.stream);
+    auto drop = [](void **buf, size_t *cap) {
+        if (*buf) hipFree(*buf);
+        *buf = nullptr;
+        *cap = 0;
+    };
+    drop(&c.cache, &c.cache_cap);
+    drop(reinterpret_cast<void **>(&c.ops), &c.ops_cap);
+    drop(&c.rops, &c.rops_cap);
+    drop(&c.maps, &c.maps_cap);
+    drop(&c.mems, &c.mems_cap);
+    for (auto &kv : jit_modules()) hipModuleUnload(kv.second.mod);
+    jit_modules().clear();
     return HGM_OK;
 }
 

@@ -205,9 +205,40 @@ def measure_fixtures():
     return out
 
 
+def shor_fixtures():
+    """Shor's circuit builders, classical helpers and whole attempts (t_CPU) of the reference."""
+    out = {"strings": [], "catalog": {}, "frac_approx": [], "mul_inv": [], "revert_bits": [], "attempts": []}
+    r = po.RefDGM(4)
+    for which, args in [("C2AddMod", [11, 0, 3, 6, 5, 10, 4, 7, 15]), ("C2SubMod", [11, 0, 2, 6, 5, 10, 4, 13, 15]),
+                        ("CMultMod", [11, 0, 1, 6, 5, 10, 4, 7, 15]), ("CRMultMod", [11, 0, 1, 6, 5, 10, 4, 7, 15]),
+                        ("CMultMod", [13, 0, 1, 7, 6, 12, 5, 11, 21]), ("CRMultMod", [13, 0, 1, 7, 6, 12, 5, 4, 21])]:
+        out["strings"].append({"which": which, "args": args, "text": r.gen(which, args)})
+    for name in r.gate_names() if hasattr(r, "gate_names") else []:
+        out["catalog"][name] = [list(map(float, (z.real, z.imag))) for z in r.get_gate(name)]
+    for w in (3, 4, 5):
+        for num in range(1, 1 << (2 * w)):
+            out["frac_approx"].append([num, 1 << (2 * w), w, *po.ref_frac_approx(num, 1 << (2 * w), w)])
+    for m in (15, 21, 35, 57, 119):
+        for v in range(2, m):
+            if np.gcd(v, m) == 1:
+                out["mul_inv"].append([v, m, po.ref_mul_inv(v, m)])
+    for bits in (4, 8, 10):
+        for v in range(0, 1 << bits, 7):
+            out["revert_bits"].append([v, bits, po.ref_revert_bits(v, bits)])
+    for number, seeds in ((15, range(1, 17)), (21, range(1, 9))):
+        for seed in seeds:
+            out["attempts"].append({"number": number, "seed": seed, "factors": list(po.ref_shor(number, seed))})
+    return out
+
+
 def main():
     if not po.ref_available():
         sys.exit("oracle/_ref/libhgmref.so is missing: run `make -C oracle ref` with the reference tree present")
+    if sys.argv[1:] == ["shor"]:
+        with open(os.path.join(HERE, "shor.json"), "w") as f:
+            json.dump(shor_fixtures(), f, indent=0)
+        print("shor.json written to", HERE)
+        return
     with open(os.path.join(HERE, "strings.json"), "w") as f:
         json.dump(gen_strings(), f, indent=0)
     with open(os.path.join(HERE, "parse.json"), "w") as f:
