@@ -244,3 +244,40 @@ def test_measure_host_path_matches_reference(golden):
         hq.seed_rng(case["seed"])
         bits = [d.measure(q) for q in range(1, n)]
         assert bits == case["bits"]
+
+
+C_PROGRAM = r"""
+#include <stdio.h>
+#include <string.h>
+#include "hgm_gpu.h"
+int main(void) {
+    hgm_tuning t; hgm_default_tuning(&t);
+    t.block_size = 64; t.repeat_count = 4; t.gpu_region_bits = 9; t.gpu_coalesced_bits = 2;
+    t.planner = HGM_PLAN_GREEDY | HGM_PLAN_FUSE | HGM_PLAN_FOLD_PHASES; t.kernel = HGM_KERNEL_REG;
+    if (hgm_validate_tuning(12, HGM_C128, &t) != HGM_OK) return 1;
+    hgm_op ops[3]; memset(ops, 0, sizeof ops);
+    const double s = 0.7071067811865476;
+    for (int i = 0; i < 3; ++i) { ops[i].target_bit = i * 4; ops[i].m[0] = s; ops[i].m[2] = s; ops[i].m[4] = s; ops[i].m[6] = -s; }
+    hgm_plan *p = NULL;
+    if (hgm_plan_build(12, HGM_C128, ops, 3, 2, &t, &p) != HGM_OK) return 2;
+    long long passes = (long long)hgm_plan_num_passes(p);
+    hgm_plan_destroy(p);
+    t.block_size = 100;
+    if (hgm_validate_tuning(12, HGM_C128, &t) != HGM_E_TUNING) return 3;
+    printf("%lld %s\n", passes, hgm_version());
+    return 0;
+}
+"""
+
+
+def test_header_is_plain_c_and_links(tmp_path):
+    """include/hgm_gpu.h compiles as C11 and a C program links libhgm.so and plans without a GPU."""
+    import subprocess
+    src = tmp_path / "t.c"
+    src.write_text(C_PROGRAM)
+    exe = tmp_path / "t"
+    lib_dir = os.path.join(ROOT, "simulador_quantico_amd")
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe), "-L", lib_dir, "-lhgm", f"-Wl,-rpath,{lib_dir}"], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert out[0] == "1" and out[1] == "hgm-mi355x"
