@@ -78,6 +78,12 @@ WORKLOADS = {
 }
 
 
+def planner_name(flags: int) -> str:
+    base = "window" if (flags & 15) == 1 else "greedy"
+    extra = [n for bit, n in ((32, "gate fusion"), (16, "phase folding")) if flags & bit]
+    return " + ".join([base] + extra)
+
+
 KERNEL_NAMES = {1: "k_pass_lds", 2: "k_pass_reg", 3: "hgm_pass_* (plan-compiled)"}
 
 
@@ -385,8 +391,7 @@ def main():
         "dtype": "f64" if dtype == np.complex128 else "f32",
         "data": "synthetic (seeded circuit, state |0..0> resident in HBM)",
         "config": {"workload": name, "qubits": n, "gates_per_step": gates, "passes_per_step": len(plan),
-                   "tile": {k: v for k, v in tun.items()}, "planner": "greedy" if tun.get("planner", 0) == 0
-                   else "window", "plan_compile_s": plan.compile_seconds, "state_norm_after": norm},
+                   "tile": {k: v for k, v in tun.items()}, "planner": planner_name(tun.get("planner", 0)), "plan_compile_s": plan.compile_seconds, "state_norm_after": norm},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": KERNEL_NAMES.get(plan.kernel, "?"),

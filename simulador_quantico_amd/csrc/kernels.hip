@@ -372,7 +372,10 @@ __global__ void __launch_bounds__(reg_kernel_max_threads(A)) k_pass_reg(typename
 // partials[tile]; k_readout_final combines the tiles of a state.
 template <class T, int A>
 __global__ void __launch_bounds__(reg_kernel_max_threads(A)) k_batch_reg(const BatchItem *__restrict__ items,
-                                                                          int n_items) {
+                                                                          int n_items,
+                                                                          const char *__restrict__ progs,
+                                                                          char *__restrict__ states,
+                                                                          char *__restrict__ parts) {
     using V = typename Vec<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double red_s[16], red_m[16];
@@ -386,10 +389,10 @@ __global__ void __launch_bounds__(reg_kernel_max_threads(A)) k_batch_reg(const B
         if (items[mid].tile_begin <= b) lo = mid; else hi = mid - 1;
     }
     const BatchItem &it = items[lo];
-    V *__restrict__ state = reinterpret_cast<V *>(it.state);
-    const RegOp *__restrict__ ops = it.ops;
-    const PhaseMap *__restrict__ maps = it.maps;
-    const PassMem *__restrict__ pm = it.pm;
+    V *__restrict__ state = reinterpret_cast<V *>(states + it.state_off);
+    const RegOp *__restrict__ ops = reinterpret_cast<const RegOp *>(progs + it.ops_off);
+    const PhaseMap *__restrict__ maps = reinterpret_cast<const PhaseMap *>(progs + it.maps_off);
+    const PassMem *__restrict__ pm = reinterpret_cast<const PassMem *>(progs + it.pm_off);
     const int n_ops = it.n_ops;
     const int64_t tid = b - it.tile_begin;
     const uint64_t base = pdep64((uint64_t)tid, it.nonlocal_mask);
@@ -412,7 +415,7 @@ __global__ void __launch_bounds__(reg_kernel_max_threads(A)) k_batch_reg(const B
     }
 #pragma unroll
     for (int k = 0; k < A; ++k) state[base | st | pm->sreg[k]] = a[k];
-    if (it.partials == nullptr) return;
+    if (it.partials_off < 0) return;
     double sum = 0.0, best = -1.0;
     uint64_t bi = ~uint64_t(0);
 #pragma unroll
@@ -443,7 +446,7 @@ __global__ void __launch_bounds__(reg_kernel_max_threads(A)) k_batch_reg(const B
         r.pmax = best;
         r.idx = bi;
         r.pad = 0;
-        it.partials[tid] = r;
+        reinterpret_cast<ReadoutPartial *>(parts + it.partials_off)[tid] = r;
     }
 }
 
@@ -642,23 +645,25 @@ hipError_t pass_reg_any(int A, void *state, const RegOp *ops, int n_ops, const P
 }
 
 template <class T, int A>
-hipError_t batch_reg(const BatchItem *items, int n_items, int64_t total_tiles, int B, size_t lds, hipStream_t s) {
+hipError_t batch_reg(const BatchItem *items, int n_items, int64_t total_tiles, int B, size_t lds, char *arena,
+                     hipStream_t s) {
     static bool attr_set = false;
     if (!attr_set) {
         hipFuncSetAttribute((const void *)k_batch_reg<T, A>, hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
         attr_set = true;
     }
-    hipLaunchKernelGGL((k_batch_reg<T, A>), dim3((unsigned)total_tiles), dim3(B), lds, s, items, n_items);
+    hipLaunchKernelGGL((k_batch_reg<T, A>), dim3((unsigned)total_tiles), dim3(B), lds, s, items, n_items,
+                       (const char *)arena, arena, arena);
     return hipGetLastError();
 }
 
 template <class T>
 hipError_t batch_reg_any(int A, const BatchItem *items, int n_items, int64_t total_tiles, int B, size_t lds,
-                         hipStream_t s) {
+                         char *arena, hipStream_t s) {
     switch (A) {
-        case 8: return batch_reg<T, 8>(items, n_items, total_tiles, B, lds, s);
-        case 16: return batch_reg<T, 16>(items, n_items, total_tiles, B, lds, s);
-        case 32: return batch_reg<T, 32>(items, n_items, total_tiles, B, lds, s);
+        case 8: return batch_reg<T, 8>(items, n_items, total_tiles, B, lds, arena, s);
+        case 16: return batch_reg<T, 16>(items, n_items, total_tiles, B, lds, arena, s);
+        case 32: return batch_reg<T, 32>(items, n_items, total_tiles, B, lds, arena, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -736,11 +741,13 @@ hipError_t launch_collapse(int dtype, void *state, uint64_t n, int bit, int outc
 }
 
 hipError_t launch_batch_reg(int dtype, int A, int B, const BatchItem *items, int n_items, int64_t total_tiles,
-                            size_t lds_bytes, hipStream_t s) {
-    if (n_items < 1 || total_tiles < 1 || total_tiles > 0x7fffffffLL || B < 1 || B > reg_kernel_max_threads(A))
+                            size_t lds_bytes, void *arena, hipStream_t s) {
+    if (n_items < 1 || total_tiles < 1 || total_tiles > 0x7fffffffLL || B < 1 || B > reg_kernel_max_threads(A) ||
+        !arena)
         return hipErrorInvalidValue;
-    return dtype == HGM_C64 ? batch_reg_any<float>(A, items, n_items, total_tiles, B, lds_bytes, s)
-                            : batch_reg_any<double>(A, items, n_items, total_tiles, B, lds_bytes, s);
+    char *base = static_cast<char *>(arena);
+    return dtype == HGM_C64 ? batch_reg_any<float>(A, items, n_items, total_tiles, B, lds_bytes, base, s)
+                            : batch_reg_any<double>(A, items, n_items, total_tiles, B, lds_bytes, base, s);
 }
 
 hipError_t launch_readout_final(const ReadoutTask *tasks, int n_tasks, hipStream_t s) {

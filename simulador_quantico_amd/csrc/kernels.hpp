@@ -36,15 +36,18 @@ struct ReadoutPartial {
 };
 static_assert(sizeof(ReadoutPartial) == 32, "shared with the kernels");
 
-// One pass of one state inside a batched register-kernel launch.
+// One pass of one state inside a batched register-kernel launch. Addresses
+// are byte offsets into the serving arena, which the kernel receives as
+// separate read-only (programs) and written (states, partials) bases, so that
+// the compiler sees global memory and reads the programs with scalar loads.
 struct BatchItem {
-    void *state;
-    const RegOp *ops;            // the pass's register program
-    const PhaseMap *maps;        // the plan's phase maps
-    const PassMem *pm;           // the pass's load/store table
+    uint64_t state_off;          // the state (2^qubits amplitudes)
+    uint64_t ops_off;            // the pass's register program (RegOp[n_ops])
+    uint64_t maps_off;           // the plan's phase maps
+    uint64_t pm_off;             // the pass's load/store table (PassMem)
     uint64_t nonlocal_mask;      // state-index bits that number the tiles
     int64_t tile_begin;          // first workgroup of this item in the launch
-    ReadoutPartial *partials;    // non-null: per-tile readout of the pass output
+    int64_t partials_off;        // >= 0: per-tile readout of the pass output (ReadoutPartial[])
     int32_t n_ops;
     int32_t pad;
 };
@@ -59,9 +62,9 @@ struct ReadoutTask {
 // One launch of the register kernel over several states (items sorted by
 // tile_begin; workgroups [0, total_tiles)); every item's plan has the same
 // A = 2*repeat_count and block size B. lds_bytes covers the largest tile of
-// an item with remaps.
+// an item with remaps; the items' offsets are relative to `arena`.
 hipError_t launch_batch_reg(int dtype, int A, int B, const BatchItem *items, int n_items, int64_t total_tiles,
-                            size_t lds_bytes, hipStream_t s);
+                            size_t lds_bytes, void *arena, hipStream_t s);
 
 // out of each task = combination of its n partials (one workgroup per task).
 hipError_t launch_readout_final(const ReadoutTask *tasks, int n_tasks, hipStream_t s);

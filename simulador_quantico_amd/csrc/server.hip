@@ -407,13 +407,14 @@ bool launch_step(hgm_server *s, Lane &L) {
         const Pass &ps = r->plan.passes[r->passes[r->next]];
         const bool last = r->next + 1 == r->passes.size();
         BatchItem &it = items[i];
-        it.state = r->d;
-        it.ops = dev_rops(r) + ps.rop_begin;
-        it.maps = dev_maps(r);
-        it.pm = dev_mems(r) + ps.mem;
+        char *const arena = s->arena.base();
+        it.state_off = (uint64_t)(r->d - arena);
+        it.ops_off = (uint64_t)(reinterpret_cast<char *>(dev_rops(r) + ps.rop_begin) - arena);
+        it.maps_off = (uint64_t)(reinterpret_cast<char *>(dev_maps(r)) - arena);
+        it.pm_off = (uint64_t)(reinterpret_cast<char *>(dev_mems(r) + ps.mem) - arena);
         it.nonlocal_mask = ((uint64_t(1) << r->qubits) - 1) & ~ps.local_mask;
         it.tile_begin = g.tiles;
-        it.partials = last ? dev_partials(r) : nullptr;
+        it.partials_off = last ? (int64_t)(reinterpret_cast<char *>(dev_partials(r)) - arena) : -1;
         it.n_ops = (int32_t)ps.rop_count;
         it.pad = 0;
         g.count++;
@@ -436,7 +437,7 @@ bool launch_step(hgm_server *s, Lane &L) {
     for (const Group &g : groups) {
         if (e != hipSuccess) break;
         e = launch_batch_reg(s->cfg.dtype, g.A, g.B, reinterpret_cast<const BatchItem *>(d) + g.first, g.count,
-                             g.tiles, g.lds, L.stream);
+                             g.tiles, g.lds, s->arena.base(), L.stream);
     }
     if (e == hipSuccess && n_tasks)
         e = launch_readout_final(reinterpret_cast<const ReadoutTask *>(d + s->tasks_off), n_tasks, L.stream);
