@@ -209,6 +209,32 @@ def fidelity_report(names=("hadamard20", "grover22", "draper29", "random30", "ra
     return out
 
 
+def drop_in_rate(name: str = "random30") -> dict:
+    """The workload through the drop-in executor (DGM with exec_type t_GPU and a
+    pinned host state; every execute() uploads the state, plans, runs the
+    passes and downloads, as the reference's t_GPU does): the PCIe-inclusive
+    rate, reported beside `value`, never as it."""
+    import simulador_quantico_amd as hq
+    w = WORKLOADS[name]
+    d, ops, iterations, basis, title = build_circuit(w)
+    d.exec_type = hq.T_GPU
+    for k, v in w["tuning"].items():
+        setattr(d, k, v)
+    d.allocateMemory()
+    d.setMemoryValue(basis)
+    d.execute(iterations)                 # warm-up: device buffer, compiled kernels
+    d.setMemoryValue(basis)
+    t0 = time.perf_counter()
+    d.execute(iterations)
+    wall = time.perf_counter() - t0
+    st = d.stats
+    gates = len(ops) * iterations
+    out = {"workload": title, "amp_updates_per_s": gates * float(1 << w["qubits"]) / wall, "wall_ms": wall * 1e3,
+           "h2d_ms": st["h2d_ms"], "kernel_ms": st["kernel_ms"], "d2h_ms": st["d2h_ms"], "plan_ms": st["plan_ms"]}
+    d.freeMemory()
+    return out
+
+
 def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026, autotune: bool = False,
                   fidelity_sample: int = 24) -> dict:
     """The serving mix on the batched server (FP32 states) against sequential
@@ -294,6 +320,8 @@ def main():
                     help="requests of the same stream run by the sequential FP64 baseline")
     ap.add_argument("--autotune", action="store_true", help="serving: autotune tile shapes per qubit count")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg of the default run")
+    ap.add_argument("--drop-in", action="store_true",
+                    help="run only the workload through the drop-in executor (PCIe-inclusive rate)")
     ap.add_argument("--no-fidelity", action="store_true", help="skip the FP32-vs-FP64 leg of the default run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain device copy of the workload's state (the known-size reference for the "
@@ -306,6 +334,10 @@ def main():
 
     import simulador_quantico_amd as hq
     from simulador_quantico_amd import _native as N
+
+    if args.drop_in:
+        print(json.dumps({"drop_in": drop_in_rate(args.workload)}))
+        return
 
     if args.fidelity:
         print(json.dumps({"fidelity_fp32_vs_fp64": fidelity_report()}))
