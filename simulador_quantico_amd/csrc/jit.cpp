@@ -237,6 +237,7 @@ std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const st
 }
 
 struct Compiled {
+    std::string source;          // compared on lookup: the hash only narrows the search
     std::vector<char> binary;
     double seconds;
 };
@@ -277,7 +278,7 @@ int jit_compile(JitCode *out, std::string *err) {
     {
         std::lock_guard<std::mutex> lk(g_cache_mu);
         auto it = g_cache.find(out->hash);
-        if (it != g_cache.end()) {
+        if (it != g_cache.end() && it->second->source == out->source) {
             out->binary = it->second->binary;
             out->compile_s = 0;
             return HGM_OK;
@@ -309,6 +310,7 @@ int jit_compile(JitCode *out, std::string *err) {
     hiprtcDestroyProgram(&prog);
     out->compile_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     auto c = std::make_shared<Compiled>();
+    c->source = out->source;
     c->binary = out->binary;
     c->seconds = out->compile_s;
     std::lock_guard<std::mutex> lk(g_cache_mu);

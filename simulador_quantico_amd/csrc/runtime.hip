@@ -69,6 +69,7 @@ struct Ctx {
 struct JitModule {
     hipModule_t mod = nullptr;
     std::vector<hipFunction_t> fns;
+    std::vector<char> binary;   // compared on lookup: the source hash only narrows the search
 };
 
 Ctx &ctx() {
@@ -131,6 +132,11 @@ int init_locked(int device) {
 int jit_load_locked(const JitCode &jc, JitModule **out) {
     auto &mods = jit_modules();
     auto it = mods.find(jc.hash);
+    if (it != mods.end() && it->second.binary != jc.binary) {   // another plan's code under the same hash
+        hipModuleUnload(it->second.mod);
+        mods.erase(it);
+        it = mods.end();
+    }
     if (it == mods.end()) {
         JitModule m;
         HGM_HIP(hipModuleLoadData(&m.mod, jc.binary.data()), "hipModuleLoadData(pass kernels)");
@@ -139,6 +145,7 @@ int jit_load_locked(const JitCode &jc, JitModule **out) {
             HGM_HIP(hipModuleGetFunction(&f, m.mod, name.c_str()), "hipModuleGetFunction");
             m.fns.push_back(f);
         }
+        m.binary = jc.binary;
         it = mods.emplace(jc.hash, std::move(m)).first;
     }
     *out = &it->second;
