@@ -70,8 +70,8 @@ WORKLOADS = {
                        tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4,
                                    planner=48)),
     "grover22": dict(qubits=22, dtype="c128", kind="grover", seed=22,
-                     tuning=dict(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=4,
-                                 kernel=2, planner=48)),
+                     tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=4,
+                                 kernel=3, planner=48)),
     "draper29": dict(qubits=29, dtype="c128", kind="draper", seed=29,
                      tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
                                  kernel=3, planner=48)),
