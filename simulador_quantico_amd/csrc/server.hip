@@ -153,8 +153,10 @@ constexpr int kServingPlanner = HGM_PLAN_GREEDY | HGM_PLAN_FUSE | HGM_PLAN_FOLD_
 
 // Tile shape the serving engine uses for a qubit count when no autotuning is
 // asked for: 16 amplitudes per thread; states up to 13 qubits are one tile;
-// larger ones use tiles of 2^12 amplitudes with 3 coalesced bits.
-hgm_tuning serving_default_tuning(int n) {
+// larger ones use tiles of 2^12 amplitudes. The always-local low bits make
+// 128-byte runs: 3 bits of 16-byte FP64 amplitudes, 4 bits of 8-byte FP32 ones.
+hgm_tuning serving_default_tuning(int n, int dtype) {
+    const int c = dtype == HGM_C64 ? 4 : 3;
     hgm_tuning t;
     hgm_default_tuning(&t);
     t.ops_block = 1 << 30;
@@ -165,12 +167,12 @@ hgm_tuning serving_default_tuning(int n) {
         t.repeat_count = (int)std::min<int64_t>(8, pairs);
         t.block_size = (int)(pairs / t.repeat_count);
         t.gpu_region_bits = n;
-        t.gpu_coalesced_bits = std::min(3, n);
+        t.gpu_coalesced_bits = std::min(c, n);
     } else {
         t.block_size = 256;
         t.repeat_count = 8;
         t.gpu_region_bits = 12;
-        t.gpu_coalesced_bits = 3;
+        t.gpu_coalesced_bits = c;
     }
     return t;
 }
@@ -576,7 +578,7 @@ int hgm_server_create(const hgm_server_config *cfg, hgm_server **out) {
     hipGetDevice(&s->device);
     s->t0 = std::chrono::steady_clock::now();
     s->tune.resize(kMaxQubits + 1);
-    for (int n = 1; n <= kMaxQubits; ++n) s->tune[n] = serving_default_tuning(n);
+    for (int n = 1; n <= kMaxQubits; ++n) s->tune[n] = serving_default_tuning(n, cfg->dtype);
     if (cfg->autotune) {
         for (int n = std::max(cfg->min_qubits, 14); n <= cfg->max_qubits; ++n) {
             const std::vector<hgm_op> ops = timing_circuit(n, 4);
