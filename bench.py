@@ -64,7 +64,7 @@ WORKLOADS = {
                        tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
                                    kernel=3, planner=48)),
     "random32f": dict(qubits=32, dtype="c64", kind="random", depth=20, seed=1,
-                      tuning=dict(block_size=512, repeat_count=4, gpu_region_bits=12, gpu_coalesced_bits=4,
+                      tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=4,
                                   kernel=3, planner=48)),
     "hadamard20": dict(qubits=20, dtype="c128", kind="hadamard", layers=1,
                        tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4,
