@@ -236,7 +236,7 @@ def drop_in_rate(name: str = "random30") -> dict:
 
 
 def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026, autotune: bool = False,
-                  fidelity_sample: int = 24) -> dict:
+                  fidelity_sample: int = 24, tile=None) -> dict:
     """The serving mix on the batched server (FP32 states) against sequential
     plain FP64 one-circuit-per-call runs of the same request stream."""
     from simulador_quantico_amd import _native as N
@@ -247,6 +247,10 @@ def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026,
     build_s = time.perf_counter() - t0
     print(f"bench: serving mix of {requests} requests built in {build_s:.1f} s", file=sys.stderr, flush=True)
     with S.Server(np.complex64, max_active=256, hbm_budget_gib=64, autotune=autotune) as srv:
+        if tile:   # multi-tile states (14 qubits and up) on another tile shape
+            B, R, r, c = tile
+            for n in range(14, 27):
+                srv.set_tuning(n, block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c)
         S.run_served(warm, srv, time_scale=0.0)
         res = S.run_served(jobs, srv)
         # fidelity of FP32 results against FP64 on a sample of the same stream (untimed)
@@ -319,6 +323,7 @@ def main():
     ap.add_argument("--seq-sample", type=int, default=256,
                     help="requests of the same stream run by the sequential FP64 baseline")
     ap.add_argument("--autotune", action="store_true", help="serving: autotune tile shapes per qubit count")
+    ap.add_argument("--serving-tile", default=None, help="serving: B,R,r,c for states of 14 qubits and more")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg of the default run")
     ap.add_argument("--drop-in", action="store_true",
                     help="run only the workload through the drop-in executor (PCIe-inclusive rate)")
@@ -344,7 +349,8 @@ def main():
         return
 
     if args.serving:
-        out = serving_bench(args.requests, args.rate, args.seq_sample, autotune=args.autotune)
+        tile = tuple(int(x) for x in args.serving_tile.split(",")) if args.serving_tile else None
+        out = serving_bench(args.requests, args.rate, args.seq_sample, autotune=args.autotune, tile=tile)
         print(json.dumps({"metric": METRIC, "value": out["circuits_per_s"], "unit": "circuits/s", "n_gpus": 1,
                           "higher_is_better": True, "vs_baseline": None, "dtype": "f32", "data": "synthetic",
                           "config": {"workload": "serving mix"}, "serving": out}))

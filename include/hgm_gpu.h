@@ -283,7 +283,11 @@ int hgm_server_poll(hgm_server *s, hgm_result *out, int cap, int *n, int timeout
 /* Block until every submitted request has finished (results stay pollable). */
 int hgm_server_wait_all(hgm_server *s);
 /* Tuning the server uses for states of `qubits` qubits. */
-int hgm_server_tuning(const hgm_server *s, int qubits, hgm_tuning *out);
+int hgm_server_tuning(hgm_server *s, int qubits, hgm_tuning *out);
+/* Replace that tuning for requests submitted afterwards (kernel must be
+ * HGM_KERNEL_REG with 2*repeat_count in {8, 16, 32}; the rest as
+ * hgm_validate_tuning checks it). */
+int hgm_server_set_tuning(hgm_server *s, int qubits, const hgm_tuning *t);
 /* Milliseconds since the server was created (the clock of hgm_result). */
 double hgm_server_clock_ms(const hgm_server *s);
 /* Finishes every submitted request, then frees the server. */

@@ -160,6 +160,7 @@ SIGNATURES = [
     ("hgm_server_poll", C.c_int, [_P, C.POINTER(Result), C.c_int, C.POINTER(C.c_int), C.c_int]),
     ("hgm_server_wait_all", C.c_int, [_P]),
     ("hgm_server_tuning", C.c_int, [_P, C.c_int, C.POINTER(Tuning)]),
+    ("hgm_server_set_tuning", C.c_int, [_P, C.c_int, C.POINTER(Tuning)]),
     ("hgm_server_clock_ms", C.c_double, [_P]),
     ("hgm_server_destroy", None, [_P]),
     ("setDevice", C.c_bool, [C.c_int]),

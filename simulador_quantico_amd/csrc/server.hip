@@ -268,14 +268,15 @@ void fail(hgm_server *s, Req *r, int status, const std::string &msg) {
 void planner_loop(hgm_server *s) {
     for (;;) {
         Req *r;
+        hgm_tuning t;
         {
             std::unique_lock<std::mutex> g(s->mu);
             s->cv_plan.wait(g, [&] { return s->stop || !s->plan_q.empty(); });
             if (s->plan_q.empty()) return;
             r = s->plan_q.front();
             s->plan_q.pop_front();
+            t = s->tune[r->qubits];
         }
-        hgm_tuning t = s->tune[r->qubits];
         std::string err;
         int rc = build_plan(r->qubits, s->cfg.dtype, r->ops.data(), (int64_t)r->ops.size(), r->iterations, t,
                             &r->plan, &err);
@@ -673,9 +674,24 @@ int hgm_server_wait_all(hgm_server *s) {
     return HGM_OK;
 }
 
-int hgm_server_tuning(const hgm_server *s, int qubits, hgm_tuning *out) {
+int hgm_server_tuning(hgm_server *s, int qubits, hgm_tuning *out) {
     if (!s || !out || qubits < 1 || qubits > kMaxQubits) { set_error("bad argument"); return HGM_E_ARG; }
+    std::lock_guard<std::mutex> g(s->mu);
     *out = s->tune[qubits];
+    return HGM_OK;
+}
+
+int hgm_server_set_tuning(hgm_server *s, int qubits, const hgm_tuning *t) {
+    if (!s || !t || qubits < 1 || qubits > kMaxQubits) { set_error("bad argument"); return HGM_E_ARG; }
+    if (t->kernel != HGM_KERNEL_REG) { set_error("the serving engine runs the register kernel (HGM_KERNEL_REG)"); return HGM_E_TUNING; }
+    TileShape shape;
+    std::string err;
+    if (int rc = make_shape(qubits, s->cfg.dtype, *t, &shape, &err)) { set_error(err); return rc; }
+    const int A = 2 * shape.R;
+    if (A != 8 && A != 16 && A != 32) { set_error("the serving kernel holds 8, 16 or 32 amplitudes per thread"); return HGM_E_TUNING; }
+    if (shape.B > reg_kernel_max_threads(A)) { set_error("block_size above the register kernel's limit for this repeat_count"); return HGM_E_TUNING; }
+    std::lock_guard<std::mutex> g(s->mu);   // planner threads copy their entry under this lock
+    s->tune[qubits] = *t;
     return HGM_OK;
 }
 

@@ -153,6 +153,15 @@ class Server:
         check(lib.hgm_server_tuning(self._h, qubits, C.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
 
+    def set_tuning(self, qubits: int, **tuning) -> None:
+        """Tile shape for requests of `qubits` qubits submitted from now on
+        (fields of hgm_tuning; kernel is the register kernel)."""
+        t = N.Tuning()
+        check(lib.hgm_server_tuning(self._h, qubits, C.byref(t)))
+        for k, v in tuning.items():
+            setattr(t, k, int(v))
+        check(lib.hgm_server_set_tuning(self._h, qubits, C.byref(t)))
+
     def clock_ms(self) -> float:
         return float(lib.hgm_server_clock_ms(self._h))
 

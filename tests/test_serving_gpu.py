@@ -112,3 +112,18 @@ def test_autotune_picks_a_valid_shape():
     assert ms.value > 0
     assert 2 * best.block_size * best.repeat_count == 1 << best.gpu_region_bits
     assert N.lib.hgm_validate_tuning(n, N.HGM_C64, C.byref(best)) == 0
+
+
+def test_tuning_override_per_qubit_count():
+    jobs = small_mix(6, seed=12, qmin=16, qmax=16)
+    with S.Server(np.complex64, max_active=16, hbm_budget_gib=1) as srv:
+        srv.set_tuning(16, block_size=512, repeat_count=4, gpu_region_bits=12, gpu_coalesced_bits=4)
+        assert srv.tuning(16)["block_size"] == 512
+        with pytest.raises(N.HGMError):
+            srv.set_tuning(16, block_size=100)
+        with pytest.raises(N.HGMError):
+            srv.set_tuning(16, kernel=N.HGM_KERNEL_JIT)
+        states, res = run_collect(srv, jobs, np.complex64, together=True)
+    for j in jobs:
+        assert res[j.index]["status"] == 0
+        assert S.fidelity(oracle_run_job(j), states[j.index]) >= 1 - 1e-6
