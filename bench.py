@@ -253,6 +253,11 @@ def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026,
                 srv.set_tuning(n, block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c)
         S.run_served(warm, srv, time_scale=0.0)
         res = S.run_served(jobs, srv)
+        t_sat = (max(r["t_done_ms"] for r in res) - min(r["t_submit_ms"] for r in res)) / 1e3
+        # the same stream offered at half the measured capacity: latency without a standing queue
+        half_rate = 0.5 * len(res) / t_sat
+        half = jobs[:min(len(jobs), 2048)]
+        res_half = S.run_served(half, srv, time_scale=rate / half_rate)
         # fidelity of FP32 results against FP64 on a sample of the same stream (untimed)
         sample = [j for j in jobs if j.qubits <= 20][:fidelity_sample]
         outs = {j.index: np.zeros(1 << j.qubits, np.complex64) for j in sample}
@@ -297,6 +302,11 @@ def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026,
                             "p99_latency_ms": float(np.percentile(seq_lat, 99)),
                             "mean_service_ms": float(service.mean()), "argmax_agree": agree},
         "speedup_circuits_per_s": served_cps / seq_cps,
+        "at_half_capacity": {
+            "arrival_rate_per_s": half_rate, "requests": len(res_half),
+            "p50_latency_ms": float(np.percentile([r["latency_ms"] for r in res_half], 50)),
+            "p99_latency_ms": float(np.percentile([r["latency_ms"] for r in res_half], 99)),
+            "failed": sum(r["status"] != 0 for r in res_half)},
         "adder_sums_exact": f"{adder_ok}/{len(adders)}",
         "fidelity_fp32_vs_fp64_min": float(min(fids)) if fids else None, "fidelity_sample": len(fids),
         "mix": "Poisson arrivals; qubits uniform in [10, 26]; uniform over Hadamard layer, Grover (1-64 "
@@ -315,6 +325,7 @@ def main():
     ap.add_argument("--ops-block", type=int, default=None)
     ap.add_argument("--tuning", default=None, help="B,R,r,c override")
     ap.add_argument("--kernel", type=int, default=None, help="0 auto, 1 LDS, 2 register, 3 plan-compiled")
+    ap.add_argument("--graph", action="store_true", help="replay plan-compiled passes from a hipGraph")
     ap.add_argument("--serving", action="store_true", help="run only the serving mix and print its JSON line")
     ap.add_argument("--fidelity", action="store_true",
                     help="run only the FP32-vs-FP64 fidelity report of the benchmark circuits")
@@ -388,6 +399,8 @@ def main():
     dtype = np.complex128 if w["dtype"] == "c128" else np.complex64
     d, ops, iterations, basis, name = build_circuit(w)
     plan = hq.Plan(n, ops, tuning, iterations, dtype)
+    if args.graph:
+        plan.set_graph(True)
     state = hq.State(n, dtype)
     state.set_basis(basis)
     print(f"bench: {name}, {len(ops) * iterations} gates, {len(plan)} passes, tuning {tun}", file=sys.stderr,

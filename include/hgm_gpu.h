@@ -205,6 +205,10 @@ int hgm_plan_maps(const hgm_plan *p, void *maps, int64_t *n_maps);
 /* Generated source of an HGM_KERNEL_JIT plan (empty for other kernels); out
  * may be NULL to query the length. */
 int hgm_plan_jit_source(const hgm_plan *p, char *out, size_t cap, size_t *len);
+/* Replay the plan's passes from a hipGraph (HGM_KERNEL_JIT plans only): the
+ * first hgm_state_run_plan on a state captures its launches, later runs on
+ * the same state replay them with one graph launch. */
+int hgm_plan_set_graph(hgm_plan *p, int enable);
 /* hipRTC compile time of an HGM_KERNEL_JIT plan in seconds (0 when the code
  * came from the process cache or the plan is not JIT). */
 double hgm_plan_compile_seconds(const hgm_plan *p);

@@ -118,6 +118,7 @@ SIGNATURES = [
     ("hgm_plan_maps", C.c_int, [_P, _P, C.POINTER(C.c_int64)]),
     ("hgm_plan_jit_source", C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     ("hgm_plan_compile_seconds", C.c_double, [_P]),
+    ("hgm_plan_set_graph", C.c_int, [_P, C.c_int]),
     ("hgm_validate_tuning", C.c_int, [C.c_int, C.c_int, C.POINTER(Tuning)]),
     ("hgm_fold_phases", C.c_int, [C.c_int, C.POINTER(Op), C.c_int64, C.POINTER(Op), C.c_int64,
                                   C.POINTER(C.c_int64)]),

@@ -35,6 +35,20 @@ struct hgm_state {
 
 struct hgm_plan {
     hgm::Plan plan;
+    // hipGraph replay of a plan-compiled plan (hgm_plan_set_graph): the graph
+    // holds the pass launches for one state pointer and one loaded module
+    bool graph_on = false;
+    mutable void *graph_state = nullptr;
+    mutable uint64_t graph_module_gen = 0;
+    mutable hipGraph_t graph = nullptr;
+    mutable hipGraphExec_t graph_exec = nullptr;
+    void drop_graph() const {
+        if (graph_exec) hipGraphExecDestroy(graph_exec);
+        if (graph) hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_state = nullptr;
+    }
 };
 
 namespace hgm {
@@ -70,7 +84,10 @@ struct JitModule {
     hipModule_t mod = nullptr;
     std::vector<hipFunction_t> fns;
     std::vector<char> binary;   // compared on lookup: the source hash only narrows the search
+    uint64_t gen = 0;           // load generation: graphs captured with an older one are stale
 };
+
+uint64_t g_module_gen = 0;
 
 Ctx &ctx() {
     static Ctx c;
@@ -146,6 +163,7 @@ int jit_load_locked(const JitCode &jc, JitModule **out) {
             m.fns.push_back(f);
         }
         m.binary = jc.binary;
+        m.gen = ++g_module_gen;
         it = mods.emplace(jc.hash, std::move(m)).first;
     }
     *out = &it->second;
@@ -176,6 +194,76 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0;
     hipEventElapsedTime(&ms, a, b);
     return ms;
+}
+
+// One pass of a plan-compiled plan: one tile per workgroup, LDS slots padded
+// by 1/16 + 1/256. Assumes the lock.
+int launch_jit_pass_locked(hgm_state *s, const Plan &p, JitModule *jm, size_t pi) {
+    Ctx &c = ctx();
+    const Pass &ps = p.passes[pi];
+    const uint64_t last = (uint64_t(1) << p.shape.r) - 1;
+    const size_t jit_lds = (size_t)(last + (last >> 4) + (last >> 8) + 1) * amp_bytes(p.dtype);
+    const int64_t n_tiles = int64_t(1) << (p.qubits - p.shape.r);
+    void *state = s->d;
+    void *args[] = {&state};
+    HGM_HIP(hipModuleLaunchKernel(jm->fns[p.jit->pass_kernel[pi]], (unsigned)n_tiles, 1, 1, (unsigned)p.shape.B, 1,
+                                  1, ps.phases > 1 ? (unsigned)jit_lds : 0u, c.stream, args, nullptr),
+            "pass launch");
+    return HGM_OK;
+}
+
+void add_stats(hgm_stats *st, const Plan &p, float kernel_ms) {
+    if (!st) return;
+    int64_t launched = 0;
+    for (const Pass &ps : p.passes) launched += ps.op_count > 0;
+    st->gates += p.gates;
+    st->passes += launched;
+    st->amp_updates += p.gates * (int64_t(1) << p.qubits);
+    st->bytes_alg += (double)launched * 2.0 * (double)(uint64_t(1) << p.qubits) * (double)amp_bytes(p.dtype);
+    st->plan_ms += p.plan_ms;
+    st->kernel_ms += kernel_ms;
+}
+
+// Replays the plan's captured graph on s, capturing it first when there is
+// none for this state and this load of the plan's kernels. Assumes the lock.
+int run_plan_graph_locked(hgm_state *s, const hgm_plan *hp, hgm_stats *st) {
+    Ctx &c = ctx();
+    const Plan &p = hp->plan;
+    if (s->qubits != p.qubits || s->dtype != p.dtype) {
+        t_err = "plan was built for another qubit count or dtype";
+        return HGM_E_STATE;
+    }
+    JitModule *jm = nullptr;
+    int rc = jit_load_locked(*p.jit, &jm);
+    if (rc) return rc;
+    if (!hp->graph_exec || hp->graph_state != s->d || hp->graph_module_gen != jm->gen) {
+        hp->drop_graph();
+        HGM_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        for (size_t pi = 0; pi < p.passes.size() && !rc; ++pi)
+            if (p.passes[pi].op_count) rc = launch_jit_pass_locked(s, p, jm, pi);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c.stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return rc ? rc : hip_fail(e, "hipStreamEndCapture");
+        }
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            hipGraphDestroy(g);
+            return hip_fail(ei, "hipGraphInstantiate");
+        }
+        hp->graph = g;
+        hp->graph_exec = x;
+        hp->graph_state = s->d;
+        hp->graph_module_gen = jm->gen;
+    }
+    HGM_HIP(hipEventRecord(c.e0, c.stream), "hipEventRecord");
+    HGM_HIP(hipGraphLaunch(hp->graph_exec, c.stream), "hipGraphLaunch");
+    HGM_HIP(hipEventRecord(c.e1, c.stream), "hipEventRecord");
+    HGM_HIP(hipEventSynchronize(c.e1), "pass execution");
+    add_stats(st, p, elapsed(c.e0, c.e1));
+    return HGM_OK;
 }
 
 // Launches every pass of the plan on the context stream. Assumes the lock.
@@ -223,15 +311,7 @@ int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st) {
         if (ps.op_count == 0) continue;
         a.nonlocal_mask = all & ~ps.local_mask;
         if (jit) {
-            // one tile per workgroup; LDS slots padded by 1/16 + 1/256
-            const uint64_t last = (uint64_t(1) << p.shape.r) - 1;
-            const size_t jit_lds = (size_t)(last + (last >> 4) + (last >> 8) + 1) * amp_bytes(p.dtype);
-            void *state = s->d;
-            void *args[] = {&state};
-            HGM_HIP(hipModuleLaunchKernel(jm->fns[p.jit->pass_kernel[pi]], (unsigned)a.n_tiles, 1, 1,
-                                          (unsigned)p.shape.B, 1, 1, ps.phases > 1 ? (unsigned)jit_lds : 0u,
-                                          c.stream, args, nullptr),
-                    "pass launch");
+            if ((rc = launch_jit_pass_locked(s, p, jm, pi))) return rc;
         } else if (reg) {
             HGM_HIP(launch_pass_reg(p.dtype, 2 * p.shape.R, s->d, static_cast<const RegOp *>(c.rops) + ps.rop_begin,
                                     (int)ps.rop_count, static_cast<const PhaseMap *>(c.maps),
@@ -245,16 +325,7 @@ int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st) {
     }
     HGM_HIP(hipEventRecord(c.e1, c.stream), "hipEventRecord");
     HGM_HIP(hipEventSynchronize(c.e1), "pass execution");
-    if (st) {
-        int64_t launched = 0;
-        for (const Pass &ps : p.passes) launched += ps.op_count > 0;
-        st->gates += p.gates;
-        st->passes += launched;
-        st->amp_updates += p.gates * (int64_t(1) << p.qubits);
-        st->bytes_alg += (double)launched * 2.0 * (double)(uint64_t(1) << p.qubits) * (double)amp_bytes(p.dtype);
-        st->plan_ms += p.plan_ms;
-        st->kernel_ms += elapsed(c.e0, c.e1);
-    }
+    add_stats(st, p, elapsed(c.e0, c.e1));
     return HGM_OK;
 }
 
@@ -559,7 +630,14 @@ int hgm_plan_build(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int 
     return HGM_OK;
 }
 
-void hgm_plan_destroy(hgm_plan *p) { delete p; }
+void hgm_plan_destroy(hgm_plan *p) {
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> g(ctx().mu);
+        p->drop_graph();
+    }
+    delete p;
+}
 
 namespace {
 int rewrite_ops(bool fuse, int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out) {
@@ -651,9 +729,21 @@ int hgm_state_run_plan(hgm_state *s, const hgm_plan *p, hgm_stats *stats) {
     auto t0 = std::chrono::steady_clock::now();
     std::lock_guard<std::mutex> g(ctx().mu);
     int rc = init_locked(-1);
-    if (!rc) rc = run_plan_locked(s, p->plan, stats);
+    if (!rc) rc = p->graph_on ? run_plan_graph_locked(s, p, stats) : run_plan_locked(s, p->plan, stats);
     if (stats) stats->total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+}
+
+int hgm_plan_set_graph(hgm_plan *p, int enable) {
+    if (!p) { t_err = "null plan"; return HGM_E_ARG; }
+    if (enable && p->plan.kernel != HGM_KERNEL_JIT) {
+        t_err = "graph replay is for plan-compiled plans (HGM_KERNEL_JIT)";
+        return HGM_E_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> g(ctx().mu);
+    if (!enable) p->drop_graph();
+    p->graph_on = enable != 0;
+    return HGM_OK;
 }
 
 int hgm_execute(void *host_state, int qubits, int dtype, const hgm_op *ops, int64_t n_ops, const hgm_tuning *tuning,

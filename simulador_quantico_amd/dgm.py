@@ -310,6 +310,10 @@ class Plan:
     def kernel(self) -> int:
         return int(lib.hgm_plan_kernel(self._h))
 
+    def set_graph(self, enable: bool = True) -> None:
+        """Replay the passes from a captured hipGraph (plan-compiled plans)."""
+        check(lib.hgm_plan_set_graph(self._h, int(enable)))
+
     @property
     def compile_seconds(self) -> float:
         return float(lib.hgm_plan_compile_seconds(self._h))

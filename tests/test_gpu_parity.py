@@ -333,3 +333,35 @@ def test_jit_random_layers_30q_round_trip():
     hq.Plan(n, inverse_ops(ops), tun).run(s)
     idx, p = s.argmax()
     assert idx == 0 and abs(p - 1) < 1e-11
+
+
+def test_jit_graph_replay_matches_direct_launches():
+    """hipGraph replay of a plan-compiled plan: bit-identical to direct launches, recaptured for another state,
+    several replays in a row with eager work in between."""
+    n = 16
+    rng = np.random.default_rng(77)
+    t, cm, cv, mats = _random_gate_list(rng, n, 120)
+    ops = hq.make_ops(t, cm, cv, mats)
+    tun = N.default_tuning(block_size=64, repeat_count=4, gpu_region_bits=9, gpu_coalesced_bits=3,
+                           kernel=N.HGM_KERNEL_JIT)
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+    direct = hq.Plan(n, ops, tun)
+    graph = hq.Plan(n, ops, tun)
+    graph.set_graph(True)
+    a, b, c = hq.State(n), hq.State(n), hq.State(n)
+    for s in (a, b, c):
+        s.upload(psi0)
+    for rep in range(3):
+        direct.run(a)
+        graph.run(b)
+        assert np.array_equal(a.download(), b.download()), rep
+        assert abs(b.norm2() - 1) < 1e-12          # eager work between replays
+    graph.run(c)                                    # another state: captured again
+    direct_c = hq.State(n)
+    direct_c.upload(psi0)
+    direct.run(direct_c)
+    assert np.array_equal(c.download(), direct_c.download())
+    with pytest.raises(N.HGMError):
+        hq.Plan(n, ops, N.default_tuning(block_size=64, repeat_count=4, gpu_region_bits=9,
+                                         kernel=N.HGM_KERNEL_REG)).set_graph(True)
