@@ -281,3 +281,21 @@ def test_header_is_plain_c_and_links(tmp_path):
                     str(exe), "-L", lib_dir, "-lhgm", f"-Wl,-rpath,{lib_dir}"], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
     assert out[0] == "1" and out[1] == "hgm-mi355x"
+
+
+def test_gpu_entry_points_fail_cleanly_without_a_device():
+    """Without a visible device the serving and autotune entry points return an error, not a crash."""
+    if N.device_count() > 0:
+        pytest.skip("a device is visible")
+    cfg = N.ServerConfig()
+    N.lib.hgm_server_default_config(C.byref(cfg))
+    h = C.c_void_p()
+    rc = N.lib.hgm_server_create(C.byref(cfg), C.byref(h))
+    assert rc != 0 and b"device" in N.lib.hgm_last_error()
+    ops = hq.make_ops([0], [0], [0], [np.eye(2)])
+    best = N.Tuning()
+    assert N.lib.hgm_autotune(8, N.HGM_C128, ops, 1, 1, N.HGM_KERNEL_REG, C.byref(best), None) != 0
+    bad = N.ServerConfig()
+    N.lib.hgm_server_default_config(C.byref(bad))
+    bad.max_active = 0
+    assert N.lib.hgm_server_create(C.byref(bad), C.byref(h)) == 1   # HGM_E_ARG: checked before any device call
