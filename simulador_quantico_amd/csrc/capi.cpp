@@ -50,14 +50,12 @@ auto visit(hgm_dgm *d, F fn) {
 }
 
 long *field(hgm::DGMCore &c, const std::string &f) {
-    static long scratch;
     if (f == "qubits") return &c.qubits;
     if (f == "thread_count") return &c.thread_count;
     if (f == "cpu_region_bits") return &c.cpu_region_bits;
     if (f == "cpu_coalesced_bits") return &c.cpu_coalesced_bits;
     if (f == "gpu_region_bits") return &c.gpu_region_bits;
     if (f == "gpu_coalesced_bits") return &c.gpu_coalesced_bits;
-    (void)scratch;
     return nullptr;
 }
 

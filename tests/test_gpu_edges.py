@@ -83,3 +83,16 @@ def test_msb_targets_and_controls_every_family():
             hq.Plan(n, ops, tun).run(s)
             say("msb", kern, planner)
             assert np.abs(s.download() - want).max() < 1e-12, (kern, planner)
+
+
+def test_allocations_beyond_hbm_fail_cleanly():
+    """A state or a serving arena larger than the device's memory is an error, and the library keeps working."""
+    with pytest.raises(N.HGMError) as e:
+        hq.State(36)                      # 1 TiB of FP64 amplitudes
+    assert e.value.code == 4              # HGM_E_NOMEM
+    from simulador_quantico_amd import serving as S
+    with pytest.raises(N.HGMError):
+        S.Server(np.complex64, hbm_budget_gib=1024)
+    s = hq.State(10)                      # and the device is still usable
+    s.set_basis(3)
+    assert s.argmax()[0] == 3
