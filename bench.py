@@ -72,6 +72,9 @@ WORKLOADS = {
     "hadamard30x3": dict(qubits=30, dtype="c128", kind="hadamard", layers=3,
                          tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
                                      kernel=3, planner=0)),
+    "hadamard30x3f": dict(qubits=30, dtype="c64", kind="hadamard", layers=3,
+                          tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=4,
+                                      kernel=3, planner=0)),
     "grover22": dict(qubits=22, dtype="c128", kind="grover", seed=22,
                      tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=4,
                                  kernel=3, planner=48)),
@@ -108,7 +111,7 @@ def measured_traffic(workload: str, tun: dict):
 def build_circuit(w):
     import simulador_quantico_amd as hq
     n = w["qubits"]
-    d = hq.DGM()
+    d = hq.DGM(np.complex64 if w["dtype"] == "c64" else np.complex128)
     d.qubits = n
     iterations = 1
     basis = 0
@@ -225,17 +228,47 @@ def drop_in_rate(name: str = "random30") -> dict:
         setattr(d, k, v)
     d.allocateMemory()
     d.setMemoryValue(basis)
-    d.execute(iterations)                 # warm-up: device buffer, compiled kernels
+    t0 = time.perf_counter()
+    d.execute(iterations)                 # first call: device buffer, kernel compilation
+    cold = time.perf_counter() - t0
     d.setMemoryValue(basis)
     t0 = time.perf_counter()
     d.execute(iterations)
     wall = time.perf_counter() - t0
     st = d.stats
     gates = len(ops) * iterations
-    out = {"workload": title, "amp_updates_per_s": gates * float(1 << w["qubits"]) / wall, "wall_ms": wall * 1e3,
-           "h2d_ms": st["h2d_ms"], "kernel_ms": st["kernel_ms"], "d2h_ms": st["d2h_ms"], "plan_ms": st["plan_ms"]}
+    out = {"workload": title, "dtype": "f64" if w["dtype"] == "c128" else "f32",
+           "amp_updates_per_s": gates * float(1 << w["qubits"]) / wall, "wall_ms": wall * 1e3,
+           "first_call_ms": cold * 1e3, "h2d_ms": st["h2d_ms"], "kernel_ms": st["kernel_ms"], "d2h_ms": st["d2h_ms"],
+           "plan_ms": st["plan_ms"]}
     d.freeMemory()
     return out
+
+
+def general_out(qubits: int, tuning: dict) -> dict:
+    """The reference's general.out timing (src/cli/general.cpp:42-52 around
+    HadamardNQubits, src/algorithms/lib_general.cpp:15-47): executor creation,
+    allocateMemory, setMemoryValue(0), setFunction(H on every qubit, 3 times),
+    execute(1) through t_GPU with a host state, and a read of state[0], in FP32
+    as the reference computes. In a fresh process this includes the HIP
+    runtime start-up, as the reference's number includes CUDA's."""
+    import simulador_quantico_amd as hq
+    t0 = time.perf_counter()
+    d = hq.DGM(np.complex64)
+    d.qubits = qubits
+    d.exec_type = hq.T_GPU
+    for k, v in tuning.items():
+        setattr(d, k, v)
+    d.allocateMemory()
+    d.setMemoryValue(0)
+    d.setFunction(hq.Hadamard(qubits, 0, qubits), 3)
+    d.execute(1)
+    amp0 = complex(d.state[0])
+    wall = time.perf_counter() - t0
+    st = d.stats
+    return {"qubits": qubits, "dtype": "f32", "tuning": tuning, "seconds": wall, "amplitude_0": [amp0.real, amp0.imag],
+            "expected_amplitude_0": 2.0 ** (-qubits / 2), "h2d_ms": st["h2d_ms"], "kernel_ms": st["kernel_ms"],
+            "d2h_ms": st["d2h_ms"], "plan_ms": st["plan_ms"], "passes": st["passes"]}
 
 
 def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026, autotune: bool = False,
@@ -339,6 +372,8 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="serving: autotune tile shapes per qubit count")
     ap.add_argument("--serving-tile", default=None, help="serving: B,R,r,c for states of 14 qubits and more")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg of the default run")
+    ap.add_argument("--general-out", type=int, default=None, metavar="QUBITS",
+                    help="time the reference's general.out (H x3, FP32, t_GPU, host state) at QUBITS qubits")
     ap.add_argument("--drop-in", action="store_true",
                     help="run only the workload through the drop-in executor (PCIe-inclusive rate)")
     ap.add_argument("--no-fidelity", action="store_true", help="skip the FP32-vs-FP64 leg of the default run")
@@ -353,6 +388,16 @@ def main():
 
     import simulador_quantico_amd as hq
     from simulador_quantico_amd import _native as N
+
+    if args.general_out:
+        tun = dict(block_size=64, repeat_count=2, gpu_region_bits=8, gpu_coalesced_bits=4)   # the CLI defaults
+        if args.tuning:
+            B, R, r, c = (int(x) for x in args.tuning.split(","))
+            tun = dict(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c)
+        if args.kernel is not None:
+            tun["kernel"] = args.kernel
+        print(json.dumps({"general_out": general_out(args.general_out, tun)}))
+        return
 
     if args.drop_in:
         print(json.dumps({"drop_in": drop_in_rate(args.workload)}))
