@@ -259,7 +259,9 @@ def general_out(qubits: int, tuning: dict) -> dict:
     d.exec_type = hq.T_GPU
     for k, v in tuning.items():
         setattr(d, k, v)
+    t1 = time.perf_counter()
     d.allocateMemory()
+    alloc_ms = (time.perf_counter() - t1) * 1e3
     d.setMemoryValue(0)
     d.setFunction(hq.Hadamard(qubits, 0, qubits), 3)
     d.execute(1)
@@ -267,7 +269,8 @@ def general_out(qubits: int, tuning: dict) -> dict:
     wall = time.perf_counter() - t0
     st = d.stats
     return {"qubits": qubits, "dtype": "f32", "tuning": tuning, "seconds": wall, "amplitude_0": [amp0.real, amp0.imag],
-            "expected_amplitude_0": 2.0 ** (-qubits / 2), "h2d_ms": st["h2d_ms"], "kernel_ms": st["kernel_ms"],
+            "expected_amplitude_0": 2.0 ** (-qubits / 2), "allocate_ms": alloc_ms, "h2d_ms": st["h2d_ms"],
+            "kernel_ms": st["kernel_ms"],
             "d2h_ms": st["d2h_ms"], "plan_ms": st["plan_ms"], "passes": st["passes"]}
 
 
