@@ -274,6 +274,29 @@ def general_out(qubits: int, tuning: dict) -> dict:
             "d2h_ms": st["d2h_ms"], "plan_ms": st["plan_ms"], "passes": st["passes"]}
 
 
+def shor_attempts(number: int, attempts: int, dtype, resident: bool, tuning: dict) -> dict:
+    """Attempts of Shor's algorithm as the reference's shor.out runs them
+    (src/cli/shor.cpp: one Shor() call timed per attempt), through t_GPU."""
+    import simulador_quantico_amd as hq
+    times, found = [], 0
+    gates = 0
+    for a in range(attempts):
+        hq.seed_rng(1000 + a)
+        t0 = time.perf_counter()
+        d = hq.DGM(dtype)
+        d.exec_type = hq.T_GPU
+        d.resident = int(resident)
+        for k, v in tuning.items():
+            setattr(d, k, v)
+        f, base, bits, st = hq.Shor(number, d)
+        times.append(time.perf_counter() - t0)
+        found += bool(f)
+        gates = st["gates"]
+    return {"number": number, "attempts": attempts, "found": found, "dtype": "f64" if dtype == np.complex128 else "f32",
+            "resident": resident, "tuning": tuning, "seconds_mean": float(np.mean(times)),
+            "seconds_first": times[0], "gates_per_attempt": gates}
+
+
 def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026, autotune: bool = False,
                   fidelity_sample: int = 24, tile=None) -> dict:
     """The serving mix on the batched server (FP32 states) against sequential
@@ -377,6 +400,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg of the default run")
     ap.add_argument("--general-out", type=int, default=None, metavar="QUBITS",
                     help="time the reference's general.out (H x3, FP32, t_GPU, host state) at QUBITS qubits")
+    ap.add_argument("--shor", type=int, default=None, metavar="N",
+                    help="time attempts of Shor's algorithm factoring N through t_GPU")
     ap.add_argument("--drop-in", action="store_true",
                     help="run only the workload through the drop-in executor (PCIe-inclusive rate)")
     ap.add_argument("--no-fidelity", action="store_true", help="skip the FP32-vs-FP64 leg of the default run")
@@ -400,6 +425,20 @@ def main():
         if args.kernel is not None:
             tun["kernel"] = args.kernel
         print(json.dumps({"general_out": general_out(args.general_out, tun)}))
+        return
+
+    if args.shor:
+        tun = dict(block_size=64, repeat_count=2, gpu_region_bits=8, gpu_coalesced_bits=4)   # the CLI defaults
+        if args.tuning:
+            B, R, r, c = (int(x) for x in args.tuning.split(","))
+            tun = dict(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c)
+        if args.kernel is not None:
+            tun["kernel"] = args.kernel
+        if args.planner is not None:
+            tun["planner"] = args.planner
+        rows = [shor_attempts(args.shor, 5, np.complex64, False, tun), shor_attempts(args.shor, 5, np.complex64, True, tun),
+                shor_attempts(args.shor, 5, np.complex128, True, tun)]
+        print(json.dumps({"shor": rows}))
         return
 
     if args.drop_in:
