@@ -56,19 +56,19 @@ def algorithmic_flops(ops, n: int, iterations: int) -> float:
 METRIC = "amp-updates/s + %HBM roofline @30q; circuits/s and p99 latency on serving mix"
 
 WORKLOADS = {
-    # name: (qubits, dtype, builder, tuning); planner 48 = greedy + gate fusion + phase folding
+    # name: (qubits, dtype, builder, tuning); planner 96 = greedy + gate fusion + row scaling
     "random30": dict(qubits=30, dtype="c128", kind="random", depth=20, seed=1,
-                     tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
-                                 kernel=3, planner=48)),
+                     tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=3,
+                                 kernel=3, planner=96)),
     "u3layers30": dict(qubits=30, dtype="c128", kind="u3layers", depth=20, seed=1,
-                       tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
-                                   kernel=3, planner=48)),
+                       tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=3,
+                                   kernel=3, planner=96)),
     "random32f": dict(qubits=32, dtype="c64", kind="random", depth=20, seed=1,
                       tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=4,
-                                  kernel=3, planner=48)),
+                                  kernel=3, planner=96)),
     "hadamard20": dict(qubits=20, dtype="c128", kind="hadamard", layers=1,
                        tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4,
-                                   planner=48)),
+                                   planner=96)),
     "hadamard30x3": dict(qubits=30, dtype="c128", kind="hadamard", layers=3,
                          tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
                                      kernel=3, planner=0)),
@@ -77,16 +77,17 @@ WORKLOADS = {
                                       kernel=3, planner=0)),
     "grover22": dict(qubits=22, dtype="c128", kind="grover", seed=22,
                      tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=4,
-                                 kernel=3, planner=48)),
+                                 kernel=3, planner=96)),
     "draper29": dict(qubits=29, dtype="c128", kind="draper", seed=29,
                      tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
-                                 kernel=3, planner=48)),
+                                 kernel=3, planner=96)),
 }
 
 
 def planner_name(flags: int) -> str:
     base = "window" if (flags & 15) == 1 else "greedy"
-    extra = [n for bit, n in ((32, "gate fusion"), (16, "phase folding")) if flags & bit]
+    extra = [n for bit, n in ((32, "gate fusion"), (64, "row scaling"), (16, "phase folding"))
+             if flags & bit and not (bit == 16 and flags & 64)]
     return " + ".join([base] + extra)
 
 
@@ -197,9 +198,9 @@ def fidelity_report(names=("hadamard20", "grover22", "draper29", "random30", "ra
         row = {"circuit": title}
         states = {}
         for dt, tun in ((np.complex128, dict(block_size=256, repeat_count=4, gpu_region_bits=11,
-                                               gpu_coalesced_bits=2, kernel=2, planner=48)),
+                                               gpu_coalesced_bits=2, kernel=2, planner=96)),
                         (np.complex64, dict(block_size=256, repeat_count=8, gpu_region_bits=12,
-                                              gpu_coalesced_bits=4, kernel=2, planner=48))):
+                                              gpu_coalesced_bits=4, kernel=2, planner=96))):
             s = hq.State(n, dt)
             s.set_basis(basis)
             st = s.apply(ops, N.default_tuning(**tun), iterations=iterations)
@@ -298,7 +299,7 @@ def shor_attempts(number: int, attempts: int, dtype, resident: bool, tuning: dic
 
 
 def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026, autotune: bool = False,
-                  fidelity_sample: int = 24, tile=None) -> dict:
+                  fidelity_sample: int = 24, tile=None, planner=None) -> dict:
     """The serving mix on the batched server (FP32 states) against sequential
     plain FP64 one-circuit-per-call runs of the same request stream."""
     from simulador_quantico_amd import _native as N
@@ -313,6 +314,9 @@ def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026,
             B, R, r, c = tile
             for n in range(14, 27):
                 srv.set_tuning(n, block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c)
+        if planner is not None:   # another gate-list rewrite for every qubit count of the mix
+            for n in range(10, 27):
+                srv.set_tuning(n, planner=planner)
         S.run_served(warm, srv, time_scale=0.0)
         res = S.run_served(jobs, srv)
         t_sat = (max(r["t_done_ms"] for r in res) - min(r["t_submit_ms"] for r in res)) / 1e3
@@ -383,7 +387,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="random30", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--planner", type=int, default=None)
+    ap.add_argument("--planner", type=int, default=None,
+                    help="HGM_PLAN_* value (with --serving: for every qubit count of the mix)")
     ap.add_argument("--ops-block", type=int, default=None)
     ap.add_argument("--tuning", default=None, help="B,R,r,c override")
     ap.add_argument("--kernel", type=int, default=None, help="0 auto, 1 LDS, 2 register, 3 plan-compiled")
@@ -451,7 +456,8 @@ def main():
 
     if args.serving:
         tile = tuple(int(x) for x in args.serving_tile.split(",")) if args.serving_tile else None
-        out = serving_bench(args.requests, args.rate, args.seq_sample, autotune=args.autotune, tile=tile)
+        out = serving_bench(args.requests, args.rate, args.seq_sample, autotune=args.autotune, tile=tile,
+                            planner=args.planner)
         print(json.dumps({"metric": METRIC, "value": out["circuits_per_s"], "unit": "circuits/s", "n_gpus": 1,
                           "higher_is_better": True, "vs_baseline": None, "dtype": "f32", "data": "synthetic",
                           "config": {"workload": "serving mix"}, "serving": out}))

@@ -66,7 +66,15 @@ enum { HGM_PLAN_GREEDY = 0, /* dependency-aware lookahead, any set of tile-local
         * placed before the first later gate on that qubit it does not commute with; products equal
         * to the identity vanish. Iterations are expanded first, so runs fuse across them. Same
         * unitary; results differ by rounding only. */
-       HGM_PLAN_FUSE = 32 };
+       HGM_PLAN_FUSE = 32,
+       /* flag: scale each row of every uncontrolled dense gate by its larger entry, which becomes
+        * exactly 1; the row factors are carried forward as a per-qubit relative factor diag(1, r)
+        * (absorbed by the next uncontrolled gate on that qubit, like uncontrolled diagonal gates)
+        * and one global factor (applied as a uniform scale when its magnitude leaves
+        * [2^-24, 2^24], and at the end). Subsumes HGM_PLAN_FOLD_PHASES. Same unitary; results
+        * differ by rounding only (a dense update drops to 8 multiply-adds per pair, 4 when the
+        * scaled matrix is real). */
+       HGM_PLAN_PIVOT = 64 };
 
 /* kernel families */
 enum { HGM_KERNEL_AUTO = 0,  /* the register kernel when the tile shape allows it, else LDS */
@@ -218,6 +226,9 @@ double hgm_plan_compile_seconds(const hgm_plan *p);
 int hgm_fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out);
 /* The gate list after fusion (HGM_PLAN_FUSE), same conventions (at most n_ops + qubits gates). */
 int hgm_fuse_gates(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out);
+/* The gate list after row scaling (HGM_PLAN_PIVOT), same conventions (at most 2 * n_ops + qubits + 1
+ * gates). */
+int hgm_pivot_rows(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out);
 
 /* Tuning check with the reference's rules plus the gfx950 limits; returns
  * HGM_E_TUNING with a message when the tuple is rejected. */

@@ -19,6 +19,7 @@ HGM_PLAN_GREEDY = 0
 HGM_PLAN_WINDOW = 1
 HGM_PLAN_FOLD_PHASES = 16
 HGM_PLAN_FUSE = 32
+HGM_PLAN_PIVOT = 64
 HGM_KERNEL_AUTO = 0
 HGM_KERNEL_LDS = 1
 HGM_KERNEL_REG = 2
@@ -123,6 +124,8 @@ SIGNATURES = [
     ("hgm_fold_phases", C.c_int, [C.c_int, C.POINTER(Op), C.c_int64, C.POINTER(Op), C.c_int64,
                                   C.POINTER(C.c_int64)]),
     ("hgm_fuse_gates", C.c_int, [C.c_int, C.POINTER(Op), C.c_int64, C.POINTER(Op), C.c_int64,
+                                 C.POINTER(C.c_int64)]),
+    ("hgm_pivot_rows", C.c_int, [C.c_int, C.POINTER(Op), C.c_int64, C.POINTER(Op), C.c_int64,
                                  C.POINTER(C.c_int64)]),
     ("hgm_dgm_create", C.c_int, [C.c_int, C.POINTER(_P)]),
     ("hgm_dgm_destroy", None, [_P]),

@@ -35,6 +35,7 @@ std::string hex32(uint32_t v) {
 
 struct Gen {
     bool f32;
+    bool split;   // remaps move real parts, then imaginary parts, through an LDS tile of scalars
     std::string V, T;
 
     double round(double v) const { return f32 ? (double)(float)v : v; }
@@ -46,16 +47,20 @@ struct Gen {
         return b;
     }
 
-    // sum of coef * var, skipping zero coefficients and folding +-1
+    // sum of coef * var, skipping zero coefficients and folding +-1. Terms
+    // with coefficient +-1 come first: the sum is evaluated left to right, so
+    // u + a * v + b * w contracts to two FMAs, while a * v + b * w + u needs a
+    // multiply, an FMA and an add.
     std::string lin(const std::vector<std::pair<double, std::string>> &terms) const {
         std::string s;
-        for (const auto &tv : terms) {
-            const double c = round(tv.first);
-            if (c == 0.0) continue;
-            if (c == 1.0) s += (s.empty() ? "" : " + ") + tv.second;
-            else if (c == -1.0) s += (s.empty() ? "-" : " - ") + tv.second;
-            else s += (s.empty() ? "" : " + ") + lit(c) + " * " + tv.second;
-        }
+        for (int unit = 1; unit >= 0; --unit)
+            for (const auto &tv : terms) {
+                const double c = round(tv.first);
+                if (c == 0.0 || (c == 1.0 || c == -1.0) != (unit == 1)) continue;
+                if (c == 1.0) s += (s.empty() ? "" : " + ") + tv.second;
+                else if (c == -1.0) s += (s.empty() ? "-" : " - ") + tv.second;
+                else s += (s.empty() ? "" : " + ") + lit(c) + " * " + tv.second;
+            }
         return s.empty() ? lit(0.0) : s;
     }
 };
@@ -121,8 +126,18 @@ std::string base_expr(uint64_t nonlocal) {
 std::string reg(int k) { return "a" + std::to_string(k); }
 
 void emit_pair(const Gen &g, std::string &o, const RegOp &op, int k0, int k1) {
+    double piv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const double *m = op.m;
-    const std::string x = "x" , y = "y";
+    if (op.kind == R_DENSE_PIV || op.kind == R_DENSE_PIV_REAL) {
+        // back to the matrix: the exact 1s and 0s vanish from the generated code
+        for (int row = 0; row < 2; ++row) {
+            const int p = (int)((op.tval >> row) & 1u);
+            piv[4 * row + 2 * p] = 1.0;
+            piv[4 * row + 2 * (1 - p)] = op.m[2 * row];
+            piv[4 * row + 2 * (1 - p) + 1] = op.m[2 * row + 1];
+        }
+        m = piv;
+    }
     o += "        { const " + g.V + " x = " + reg(k0) + ", y = " + reg(k1) + ";\n";
     if (op.kind == R_SWAP) {
         o += "          " + reg(k0) + " = y; " + reg(k1) + " = x; }\n";
@@ -175,7 +190,7 @@ std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const st
     std::string o;
     o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(B) + ") " + name + "(" + g.V +
          " *__restrict__ s) {\n";
-    if (ps.phases > 1) o += "    extern __shared__ " + g.V + " lds[];\n";
+    if (ps.phases > 1) o += "    extern __shared__ " + (g.split ? g.T : g.V) + " lds[];\n";
     o += "    const unsigned t = threadIdx.x;\n";
     o += "    const unsigned long long lt = " + thread_or(lt, tb) + ";\n";
     o += "    const unsigned long long st = " + thread_or(st, tb) + ";\n";
@@ -201,14 +216,18 @@ std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const st
         const RegOp &op = ops[i];
         if (op.kind == R_REMAP) {
             const int nxt = op.rbit;
-            o += "        __syncthreads();\n";
-            for (int k = 0; k < A; ++k)
-                o += "        lds[f" + std::to_string(cur - m_first) + " + " +
-                     std::to_string(padded(unswizzle(plan.maps[cur].reg[k]))) + "u] = " + reg(k) + ";\n";
-            o += "        __syncthreads();\n";
-            for (int k = 0; k < A; ++k)
-                o += "        " + reg(k) + " = lds[f" + std::to_string(nxt - m_first) + " + " +
-                     std::to_string(padded(unswizzle(plan.maps[nxt].reg[k]))) + "u];\n";
+            const char *parts[2][2] = {{"", nullptr}, {".x", ".y"}};
+            for (int h = 0; h < 2 && parts[g.split][h]; ++h) {
+                const std::string part = parts[g.split][h];
+                o += "        __syncthreads();\n";
+                for (int k = 0; k < A; ++k)
+                    o += "        lds[f" + std::to_string(cur - m_first) + " + " +
+                         std::to_string(padded(unswizzle(plan.maps[cur].reg[k]))) + "u] = " + reg(k) + part + ";\n";
+                o += "        __syncthreads();\n";
+                for (int k = 0; k < A; ++k)
+                    o += "        " + reg(k) + part + " = lds[f" + std::to_string(nxt - m_first) + " + " +
+                         std::to_string(padded(unswizzle(plan.maps[nxt].reg[k]))) + "u];\n";
+            }
             cur = nxt;
             continue;
         }
@@ -252,6 +271,13 @@ void jit_generate(const Plan &plan, JitCode *out) {
     g.f32 = plan.dtype == HGM_C64;
     g.V = g.f32 ? "hgm_f2" : "hgm_d2";
     g.T = g.f32 ? "float" : "double";
+    // LDS per workgroup caps the workgroups a CU holds (160 KiB each): a tile
+    // above 32 KiB moves through LDS in two halves of scalars instead, which
+    // doubles the resident workgroups for two more barriers per remap
+    const uint64_t last = (uint64_t(1) << plan.shape.r) - 1;
+    const size_t entries = (size_t)padded((uint32_t)last) + 1;
+    g.split = entries * amp_bytes(plan.dtype) > 32 * 1024;
+    out->lds_bytes = entries * amp_bytes(plan.dtype) / (g.split ? 2 : 1);
     out->source = "// generated pass kernels\n"
                   "struct __attribute__((aligned(16))) hgm_d2 { double x, y; };\n"
                   "struct __attribute__((aligned(8))) hgm_f2 { float x, y; };\n";

@@ -21,6 +21,7 @@ struct JitCode {
     std::vector<char> binary;    // gfx950 code object
     std::vector<int32_t> pass_kernel;   // kernel index of each pass (passes with identical code share one)
     std::vector<std::string> names;     // kernel symbol names
+    size_t lds_bytes = 0;        // dynamic LDS of a pass kernel with remaps
     uint64_t hash = 0;           // of the source
     double compile_s = 0;
 };

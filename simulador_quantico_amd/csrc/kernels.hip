@@ -199,6 +199,63 @@ __device__ __forceinline__ void reg_dense_real(typename Vec<T>::type (&a)[A], co
     }
 }
 
+// dispatch on the runtime register bit (tile-uniform) to unrolled code
+#define HGM_REG_BITS(CALL)                                  \
+    switch (bit) {                                          \
+        case 0: CALL(0); break;                             \
+        case 1: if constexpr (A > 2) { CALL(1); } break;    \
+        case 2: if constexpr (A > 4) { CALL(2); } break;    \
+        case 3: if constexpr (A > 8) { CALL(3); } break;    \
+        case 4: if constexpr (A > 16) { CALL(4); } break;   \
+        default: break;                                     \
+    }
+
+// Rows with an exact unit entry (R_DENSE_PIV*): out_i = x[P_i] + t_i * x[1 - P_i],
+// 8 multiply-adds per pair, 4 when t0 and t1 are real.
+template <class T, int A, int BIT, int P0, int P1, bool REAL>
+__device__ __forceinline__ void reg_dense_piv(typename Vec<T>::type (&a)[A], T t0r, T t0i, T t1r, T t1i) {
+    using V = typename Vec<T>::type;
+#pragma unroll
+    for (int k0 = 0; k0 < A; ++k0) {
+        if (k0 & (1 << BIT)) continue;
+        const int k1 = k0 | (1 << BIT);
+        const V x[2] = {a[k0], a[k1]};
+        V p, q;
+        if constexpr (REAL) {
+            p.x = x[P0].x + t0r * x[1 - P0].x;
+            p.y = x[P0].y + t0r * x[1 - P0].y;
+            q.x = x[P1].x + t1r * x[1 - P1].x;
+            q.y = x[P1].y + t1r * x[1 - P1].y;
+        } else {
+            p.x = x[P0].x + t0r * x[1 - P0].x - t0i * x[1 - P0].y;
+            p.y = x[P0].y + t0r * x[1 - P0].y + t0i * x[1 - P0].x;
+            q.x = x[P1].x + t1r * x[1 - P1].x - t1i * x[1 - P1].y;
+            q.y = x[P1].y + t1r * x[1 - P1].y + t1i * x[1 - P1].x;
+        }
+        a[k0] = p;
+        a[k1] = q;
+    }
+}
+
+template <class T, int A, int P0, int P1, bool REAL>
+__device__ __forceinline__ void reg_dense_piv_bits(typename Vec<T>::type (&a)[A], int bit, T t0r, T t0i, T t1r,
+                                                   T t1i) {
+#define HGM_CALL(B) reg_dense_piv<T, A, B, P0, P1, REAL>(a, t0r, t0i, t1r, t1i)
+    HGM_REG_BITS(HGM_CALL)
+#undef HGM_CALL
+}
+
+template <class T, int A, bool REAL>
+__device__ __forceinline__ void reg_dense_piv_any(typename Vec<T>::type (&a)[A], const RegOp &o) {
+    const T t0r = (T)o.m[0], t0i = (T)o.m[1], t1r = (T)o.m[2], t1i = (T)o.m[3];
+    switch (o.tval & 3u) {
+        case 0: reg_dense_piv_bits<T, A, 0, 0, REAL>(a, o.rbit, t0r, t0i, t1r, t1i); break;
+        case 1: reg_dense_piv_bits<T, A, 1, 0, REAL>(a, o.rbit, t0r, t0i, t1r, t1i); break;
+        case 2: reg_dense_piv_bits<T, A, 0, 1, REAL>(a, o.rbit, t0r, t0i, t1r, t1i); break;
+        default: reg_dense_piv_bits<T, A, 1, 1, REAL>(a, o.rbit, t0r, t0i, t1r, t1i); break;
+    }
+}
+
 template <class T, int A, int BIT>
 __device__ __forceinline__ void reg_anti(typename Vec<T>::type (&a)[A], const Mat<T> &m) {
     using V = typename Vec<T>::type;
@@ -225,20 +282,11 @@ __device__ __forceinline__ void reg_swap(typename Vec<T>::type (&a)[A]) {
     }
 }
 
-// dispatch on the runtime register bit (tile-uniform) to unrolled code
-#define HGM_REG_BITS(CALL)                                  \
-    switch (bit) {                                          \
-        case 0: CALL(0); break;                             \
-        case 1: if constexpr (A > 2) { CALL(1); } break;    \
-        case 2: if constexpr (A > 4) { CALL(2); } break;    \
-        case 3: if constexpr (A > 8) { CALL(3); } break;    \
-        case 4: if constexpr (A > 16) { CALL(4); } break;   \
-        default: break;                                     \
-    }
-
 template <class T, int A>
 __device__ __forceinline__ void reg_gate(typename Vec<T>::type (&a)[A], const RegOp &o, int t) {
     const int bit = o.rbit;
+    if (o.kind == R_DENSE_PIV) { reg_dense_piv_any<T, A, false>(a, o); return; }
+    if (o.kind == R_DENSE_PIV_REAL) { reg_dense_piv_any<T, A, true>(a, o); return; }
     if (o.kind == R_SWAP) {
 #define HGM_CALL(B) reg_swap<T, A, B>(a)
         HGM_REG_BITS(HGM_CALL)
@@ -596,11 +644,20 @@ __global__ void k_collapse(typename Vec<T>::type *state, uint64_t half, int bit,
     }
 }
 
+uint64_t max_blocks(int block) { return 0xffffffffull / (uint64_t)block; }
+
 int grid_for(uint64_t n, int block) {
     uint64_t g = (n + block - 1) / block;
     if (g > (uint64_t(1) << 24)) g = uint64_t(1) << 24;
+    if (g > max_blocks(block)) g = max_blocks(block);
     if (g < 1) g = 1;
     return (int)g;
+}
+
+int64_t tile_grid(int64_t n_tiles, int block) {
+    int64_t g = n_tiles < (int64_t(1) << 22) ? n_tiles : (int64_t(1) << 22);
+    if ((uint64_t)g > max_blocks(block)) g = (int64_t)max_blocks(block);
+    return g;
 }
 
 template <class T>
@@ -611,7 +668,7 @@ hipError_t pass_lds(void *state, const DevOp *ops, int n_ops, const PassArgs &a,
         attr_set = true;
     }
     const size_t lds = 3 * 64 * sizeof(uint64_t) + (size_t(1) << a.r) * sizeof(typename Vec<T>::type);
-    int64_t grid = a.n_tiles < (int64_t(1) << 22) ? a.n_tiles : (int64_t(1) << 22);
+    const int64_t grid = tile_grid(a.n_tiles, a.B);
     hipLaunchKernelGGL(k_pass_lds<T>, dim3((unsigned)grid), dim3(a.B), lds, s,
                        reinterpret_cast<typename Vec<T>::type *>(state), ops, n_ops, a);
     return hipGetLastError();
@@ -625,7 +682,7 @@ hipError_t pass_reg(void *state, const RegOp *ops, int n_ops, const PhaseMap *ma
         hipFuncSetAttribute((const void *)k_pass_reg<T, A>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
-    int64_t grid = n_tiles < (int64_t(1) << 22) ? n_tiles : (int64_t(1) << 22);
+    const int64_t grid = tile_grid(n_tiles, B);
     hipLaunchKernelGGL((k_pass_reg<T, A>), dim3((unsigned)grid), dim3(B), lds, s,
                        reinterpret_cast<typename Vec<T>::type *>(state), ops, n_ops, maps, pm, nonlocal, n_tiles, r);
     return hipGetLastError();
@@ -687,7 +744,7 @@ hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) 
     const uint64_t n16 = bytes / 16;
     const int blk = 256;
     uint64_t g = (n16 + blk - 1) / blk;
-    if (g > (uint64_t(1) << 30)) g = uint64_t(1) << 30;
+    if (g > max_blocks(blk)) g = max_blocks(blk);
     if (g < 1) g = 1;
     hipLaunchKernelGGL(k_copy, dim3((unsigned)g), dim3(blk), 0, s, (uint4 *)dst, (const uint4 *)src, n16);
     return hipGetLastError();
@@ -744,7 +801,7 @@ hipError_t launch_collapse(int dtype, void *state, uint64_t n, int bit, int outc
 
 hipError_t launch_batch_reg(int dtype, int A, int B, const BatchItem *items, int n_items, int64_t total_tiles,
                             size_t lds_bytes, void *arena, hipStream_t s) {
-    if (n_items < 1 || total_tiles < 1 || total_tiles > 0x7fffffffLL || B < 1 || B > reg_kernel_max_threads(A) ||
+    if (n_items < 1 || total_tiles < 1 || B < 1 || (uint64_t)total_tiles > max_blocks(B) || B > reg_kernel_max_threads(A) ||
         !arena)
         return hipErrorInvalidValue;
     char *base = static_cast<char *>(arena);

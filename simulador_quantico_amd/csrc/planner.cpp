@@ -114,7 +114,7 @@ int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::
     if (B > 1024) { *err = "Erro de tuning: block_size must be <= 1024 on gfx950"; return HGM_E_TUNING; }
     if (!is_pow2(B) || !is_pow2(R)) { *err = "Erro de tuning: block_size and repeat_count must be powers of two"; return HGM_E_TUNING; }
     if (t.ops_block < 1) { *err = "Erro de tuning: ops_block must be >= 1"; return HGM_E_TUNING; }
-    const int base_planner = t.planner & ~(HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE);
+    const int base_planner = t.planner & ~(HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE | HGM_PLAN_PIVOT);
     if (base_planner != HGM_PLAN_GREEDY && base_planner != HGM_PLAN_WINDOW) {
         *err = "unknown planner";
         return HGM_E_TUNING;
@@ -249,6 +249,90 @@ void fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_o
     for (int b = 0; b < qubits; ++b) flush(b);
 }
 
+void pivot_rows(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op> *out) {
+    using C = std::complex<double>;
+    // What is still to be applied after the emitted list: g * prod_b diag(1, r[b]) on bit b.
+    std::vector<C> r((size_t)qubits, C(1, 0));
+    C g(1, 0);
+    auto get = [](const hgm_op &o, int k) { return C(o.m[2 * k], o.m[2 * k + 1]); };
+    auto set = [](hgm_op &o, int k, C v) { o.m[2 * k] = v.real(); o.m[2 * k + 1] = v.imag(); };
+    auto diag_op = [&](int b, C a, C d) {
+        hgm_op o{};
+        o.target_bit = b;
+        set(o, 0, a);
+        set(o, 3, d);
+        out->push_back(o);
+    };
+    auto flush = [&](int b) {
+        if (r[b] == C(1, 0)) return;
+        diag_op(b, C(1, 0), r[b]);
+        r[b] = C(1, 0);
+    };
+    // the emitted amplitudes are the true ones divided by g: keep |g| near 1
+    // so that FP32 states neither overflow nor lose small amplitudes
+    auto rescale = [&]() {
+        const double s = std::abs(g);
+        if (s >= 0x1p-24 && s <= 0x1p24) return;
+        diag_op(0, C(s, 0), C(s, 0));
+        g /= s;
+    };
+    out->clear();
+    out->reserve((size_t)n_ops + (size_t)qubits + 1);
+    for (int64_t i = 0; i < n_ops; ++i) {
+        hgm_op o = ops[i];
+        const int b = o.target_bit;
+        const int kind = classify(o.m);
+        const C u00 = get(o, 0), u11 = get(o, 3);
+        if (kind == K_DIAG) {
+            if (o.control_mask || u00 == C(0, 0)) {   // commutes with every carried factor
+                out->push_back(o);
+                continue;
+            }
+            g *= u00;                                  // diag(u00, u11) = u00 * diag(1, u11 / u00)
+            r[b] *= u11 / u00;
+            rescale();
+            continue;
+        }
+        if (o.control_mask) {            // the factor on b must act on both control halves first
+            flush(b);
+            out->push_back(o);
+            continue;
+        }
+        // U' = U * diag(1, r): each row scaled by its larger entry, which becomes exactly 1
+        const C v[2][2] = {{u00, get(o, 1) * r[b]}, {get(o, 2), u11 * r[b]}};
+        int p[2];
+        C d[2];
+        for (int row = 0; row < 2; ++row) {
+            p[row] = std::abs(v[row][1]) > std::abs(v[row][0]) ? 1 : 0;
+            d[row] = v[row][p[row]];
+        }
+        if (d[0] == C(0, 0) || d[1] == C(0, 0)) {   // a zero row: emit U' itself
+            set(o, 0, v[0][0]);
+            set(o, 1, v[0][1]);
+            set(o, 2, v[1][0]);
+            set(o, 3, v[1][1]);
+            r[b] = C(1, 0);
+            out->push_back(o);
+            continue;
+        }
+        for (int row = 0; row < 2; ++row) {
+            set(o, 2 * row + p[row], C(1, 0));
+            set(o, 2 * row + 1 - p[row], v[row][1 - p[row]] / d[row]);
+        }
+        out->push_back(o);
+        g *= d[0];                       // U' = diag(d0, d1) * M = d0 * diag(1, d1 / d0) * M
+        r[b] = d[1] / d[0];
+        rescale();
+    }
+    for (int b = 0; b < qubits; ++b) {
+        if (r[b] == C(1, 0)) continue;
+        diag_op(b, g, g * r[b]);         // the global factor rides on the first remaining diagonal
+        g = C(1, 0);
+        r[b] = C(1, 0);
+    }
+    if (g != C(1, 0)) diag_op(0, g, g);
+}
+
 int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, const hgm_tuning &t,
                Plan *out, std::string *err) {
     auto t0 = std::chrono::steady_clock::now();
@@ -258,7 +342,7 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
     int st = make_shape(qubits, dtype, t, &plan.shape, err);
     if (st) return st;
     std::vector<hgm_op> expanded, fused, folded;
-    if ((t.planner & (HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE)) && n_ops > 0 && ops) {
+    if ((t.planner & (HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE | HGM_PLAN_PIVOT)) && n_ops > 0 && ops) {
         for (int64_t i = 0; i < n_ops; ++i)
             if (ops[i].target_bit < 0 || ops[i].target_bit >= qubits || (ops[i].control_mask >> ops[i].target_bit) & 1) {
                 *err = "gate " + std::to_string(i) + ": target bit " + std::to_string(ops[i].target_bit) +
@@ -277,7 +361,11 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
             ops = fused.data();
             n_ops = (int64_t)fused.size();
         }
-        if (t.planner & HGM_PLAN_FOLD_PHASES) {
+        if (t.planner & HGM_PLAN_PIVOT) {   // subsumes phase folding
+            pivot_rows(qubits, ops, n_ops, &folded);
+            ops = folded.data();
+            n_ops = (int64_t)folded.size();
+        } else if (t.planner & HGM_PLAN_FOLD_PHASES) {
             fold_phases(qubits, ops, n_ops, &folded);
             ops = folded.data();
             n_ops = (int64_t)folded.size();
@@ -426,6 +514,13 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
         plan.kernel = t.kernel == HGM_KERNEL_JIT ? HGM_KERNEL_JIT : HGM_KERNEL_REG;
         build_register_program(plan);
         if (plan.kernel == HGM_KERNEL_JIT) {
+            // one workgroup per tile: a dispatch holds at most 2^32 - 1 work-items
+            const uint64_t items = (uint64_t(1) << (qubits - plan.shape.r)) * (uint64_t)plan.shape.B;
+            if (items > 0xffffffffull) {
+                *err = "Erro de tuning: a plan-compiled pass launches 2^(qubits-1)/repeat_count work-items, above "
+                       "the 2^32 - 1 of one dispatch; raise repeat_count";
+                return HGM_E_TUNING;
+            }
             plan.jit = std::make_shared<JitCode>();
             jit_generate(plan, plan.jit.get());
             if (int rc = jit_compile(plan.jit.get(), err)) return rc;
@@ -505,6 +600,14 @@ uint64_t to_state(uint32_t tile_mask, const Pass &ps) {
 }
 
 bool is_one(double re, double im) { return re == 1.0 && im == 0.0; }
+
+// Column of row `row` of the row-major 2x2 matrix m that holds an exact 1
+// (the first such column).
+bool unit_pivot(const double m[8], int row, int *col) {
+    for (int c = 0; c < 2; ++c)
+        if (is_one(m[4 * row + 2 * c], m[4 * row + 2 * c + 1])) { *col = c; return true; }
+    return false;
+}
 
 }  // namespace
 
@@ -606,6 +709,16 @@ void build_register_program(Plan &plan) {
                     o.gmask = d.gmask;
                     o.gval = d.gval;
                     std::memcpy(o.m, d.m, sizeof(o.m));
+                    int p0, p1;
+                    if (d.kind == K_DENSE && !o.rmask && !o.tmask && unit_pivot(d.m, 0, &p0) &&
+                        unit_pivot(d.m, 1, &p1)) {
+                        const double *t0 = d.m + 2 * (1 - p0), *t1 = d.m + 4 + 2 * (1 - p1);
+                        o.kind = (t0[1] == 0.0 && t1[1] == 0.0) ? R_DENSE_PIV_REAL : R_DENSE_PIV;
+                        o.tval = (uint32_t)(p0 | (p1 << 1));
+                        const double m4[4] = {t0[0], t0[1], t1[0], t1[1]};
+                        std::memset(o.m, 0, sizeof(o.m));
+                        std::memcpy(o.m, m4, sizeof(m4));
+                    }
                     plan.rops.push_back(o);
                     continue;
                 }

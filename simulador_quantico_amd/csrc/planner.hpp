@@ -88,7 +88,13 @@ enum RegKind : int32_t {
     R_REMAP = 3,       // move to phase map `rbit` through LDS
     R_SWAP = 4,        // exchange of the pair (the X matrix)
     R_PHASE_REAL = 5,  // R_PHASE with a real factor m[0] (m[1] == 0)
-    R_DENSE_REAL = 6   // R_DENSE with a real matrix (imaginary parts exactly 0)
+    R_DENSE_REAL = 6,  // R_DENSE with a real matrix (imaginary parts exactly 0)
+    // Unconditioned 2x2 update whose rows each hold an exact 1 (HGM_PLAN_PIVOT):
+    //   out0 = x[p0] + t0 * x[1 - p0],  out1 = x[p1] + t1 * x[1 - p1]
+    // with (x[0], x[1]) the pair, t0 = (m[0], m[1]), t1 = (m[2], m[3]) and the
+    // pivot columns p0 = tval & 1, p1 = tval >> 1 (rmask == tmask == 0).
+    R_DENSE_PIV = 7,
+    R_DENSE_PIV_REAL = 8   // R_DENSE_PIV with real t0, t1
 };
 
 struct RegOp {
@@ -97,7 +103,7 @@ struct RegOp {
     uint32_t rmask;     // condition on register-index bits
     uint32_t rval;
     uint32_t tmask;     // condition on thread-index bits
-    uint32_t tval;
+    uint32_t tval;      // (the pivot columns for R_DENSE_PIV*)
     uint64_t gmask;     // condition on non-local state-index bits
     uint64_t gval;
     double m[8];
@@ -162,6 +168,19 @@ int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::
 // of 16 (14 when m00 is already real).
 void fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op> *out);
 
+// Equivalent gate list in which each row of every uncontrolled dense gate is
+// divided by its larger-magnitude entry, so that entry is exactly 1 and the
+// other has magnitude <= 1. The row factors d0, d1 are carried forward as
+// d0 * diag(1, d1 / d0): the relative factor stays on the qubit until the next
+// uncontrolled gate there absorbs it (uncontrolled diagonal gates are absorbed
+// into it; controls commute with it; a controlled non-diagonal gate on the
+// qubit gets it as a diagonal gate first), and d0 joins one global factor g.
+// The emitted list therefore produces the state divided by g; a uniform scale
+// by |g| is emitted whenever |g| leaves [2^-24, 2^24], and g itself with the
+// last carried diagonal. A dense update then costs 8 real multiply-adds per
+// amplitude pair (4 when the scaled matrix is real) instead of 16.
+void pivot_rows(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op> *out);
+
 // Equivalent gate list in which every run of uncontrolled gates on one qubit
 // (with nothing in between that touches the qubit, except diagonal gates
 // while the run so far is diagonal) is one gate: the product of the run,
@@ -170,8 +189,9 @@ void fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_o
 void fuse_gates(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op> *out);
 
 // Builds the pass list for ops[0..n_ops) repeated `iterations` times
-// (after fuse_gates and fold_phases when tuning.planner has HGM_PLAN_FUSE /
-// HGM_PLAN_FOLD_PHASES; fusing first expands the iterations).
+// (after fuse_gates, then pivot_rows or fold_phases, when tuning.planner has
+// HGM_PLAN_FUSE, HGM_PLAN_PIVOT or HGM_PLAN_FOLD_PHASES; fusing first expands
+// the iterations).
 int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, const hgm_tuning &t,
                Plan *out, std::string *err);
 

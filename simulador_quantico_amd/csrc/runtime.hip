@@ -105,6 +105,7 @@ std::unordered_map<uint64_t, JitModule> &jit_modules() {
 
 int hip_fail(hipError_t e, const char *what) {
     t_err = std::string(what) + ": " + hipGetErrorString(e);
+    (void)hipGetLastError();   // a failed call leaves its error for the next launch check to find
     return HGM_E_HIP;
 }
 
@@ -203,8 +204,7 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 int launch_jit_pass_locked(hgm_state *s, const Plan &p, JitModule *jm, size_t pi) {
     Ctx &c = ctx();
     const Pass &ps = p.passes[pi];
-    const uint64_t last = (uint64_t(1) << p.shape.r) - 1;
-    const size_t jit_lds = (size_t)(last + (last >> 4) + (last >> 8) + 1) * amp_bytes(p.dtype);
+    const size_t jit_lds = p.jit->lds_bytes;
     const int64_t n_tiles = int64_t(1) << (p.qubits - p.shape.r);
     void *state = s->d;
     void *args[] = {&state};
@@ -306,7 +306,8 @@ int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st) {
     a.n_tiles = int64_t(1) << (p.qubits - p.shape.r);
     const uint64_t all = (uint64_t(1) << p.qubits) - 1;
     const size_t tile_bytes = (size_t(1) << p.shape.r) * amp_bytes(p.dtype);
-    const unsigned grid = (unsigned)(a.n_tiles < (int64_t(1) << 22) ? a.n_tiles : (int64_t(1) << 22));
+    // grid-stride over tiles; a dispatch holds at most 2^32 - 1 work-items
+    const unsigned grid = (unsigned)std::min<int64_t>({a.n_tiles, int64_t(1) << 22, (int64_t)(0xffffffffu / (unsigned)a.B)});
     HGM_HIP(hipEventRecord(c.e0, c.stream), "hipEventRecord");
     for (size_t pi = 0; pi < p.passes.size(); ++pi) {
         const Pass &ps = p.passes[pi];
@@ -364,6 +365,7 @@ int gpu_init(int device) {
 int alloc_pinned(void **p, size_t bytes) {
     if (hipHostMalloc(p, bytes, hipHostMallocDefault) != hipSuccess) {
         *p = nullptr;
+        (void)hipGetLastError();
         return HGM_E_NOMEM;
     }
     // zero-fill like calloc; large states in parallel chunks (one thread
@@ -660,7 +662,9 @@ void hgm_plan_destroy(hgm_plan *p) {
 }
 
 namespace {
-int rewrite_ops(bool fuse, int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out) {
+enum class Rewrite { fold, fuse, pivot };
+
+int rewrite_ops(Rewrite how, int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out) {
     if (!n_out || n_ops < 0 || (n_ops > 0 && !ops) || qubits < 1 || qubits > kMaxQubits) {
         t_err = "bad argument";
         return HGM_E_ARG;
@@ -671,7 +675,8 @@ int rewrite_ops(bool fuse, int qubits, const hgm_op *ops, int64_t n_ops, hgm_op 
             return HGM_E_ARG;
         }
     std::vector<hgm_op> v;
-    if (fuse) fuse_gates(qubits, ops, n_ops, &v);
+    if (how == Rewrite::fuse) fuse_gates(qubits, ops, n_ops, &v);
+    else if (how == Rewrite::pivot) pivot_rows(qubits, ops, n_ops, &v);
     else fold_phases(qubits, ops, n_ops, &v);
     *n_out = (int64_t)v.size();
     if (out) {
@@ -683,11 +688,15 @@ int rewrite_ops(bool fuse, int qubits, const hgm_op *ops, int64_t n_ops, hgm_op 
 }  // namespace
 
 int hgm_fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out) {
-    return rewrite_ops(false, qubits, ops, n_ops, out, cap, n_out);
+    return rewrite_ops(Rewrite::fold, qubits, ops, n_ops, out, cap, n_out);
 }
 
 int hgm_fuse_gates(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out) {
-    return rewrite_ops(true, qubits, ops, n_ops, out, cap, n_out);
+    return rewrite_ops(Rewrite::fuse, qubits, ops, n_ops, out, cap, n_out);
+}
+
+int hgm_pivot_rows(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out) {
+    return rewrite_ops(Rewrite::pivot, qubits, ops, n_ops, out, cap, n_out);
 }
 
 int64_t hgm_plan_num_passes(const hgm_plan *p) { return p ? (int64_t)p->plan.passes.size() : -1; }

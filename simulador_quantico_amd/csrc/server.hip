@@ -37,7 +37,12 @@ namespace {
 constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
-std::string hip_msg(hipError_t e, const char *what) { return std::string(what) + ": " + hipGetErrorString(e); }
+// Message of a failed HIP call; also clears the error, which the next
+// launch check (hipGetLastError) would otherwise report as its own.
+std::string hip_msg(hipError_t e, const char *what) {
+    (void)hipGetLastError();
+    return std::string(what) + ": " + hipGetErrorString(e);
+}
 
 // First-fit allocator over one device (or pinned host) allocation; freed
 // ranges merge with their neighbours.
@@ -147,9 +152,10 @@ double now_ms(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-// Gate lists are fused and phase-folded before planning (same unitary, fewer
-// and real-valued gates: Grover's X-H-X runs become one real gate per qubit).
-constexpr int kServingPlanner = HGM_PLAN_GREEDY | HGM_PLAN_FUSE | HGM_PLAN_FOLD_PHASES;
+// Gate lists are fused and row-scaled before planning (same unitary, fewer
+// and cheaper gates: Grover's X-H-X runs become one real gate per qubit, and
+// each dense update costs 8 multiply-adds per pair, 4 when real).
+constexpr int kServingPlanner = HGM_PLAN_GREEDY | HGM_PLAN_FUSE | HGM_PLAN_PIVOT;
 
 // Tile shape the serving engine uses for a qubit count when no autotuning is
 // asked for: 16 amplitudes per thread; states up to 13 qubits are one tile;

@@ -2,7 +2,8 @@
 run on the oracle equals the original within FP64 rounding, every uncontrolled
 dense gate of the folded list has a real non-negative first column, and the
 list grows only by the carried diagonal factors: one before a controlled
-non-diagonal gate on a qubit that carries one, and one per qubit at the end."""
+non-diagonal gate on a qubit that carries one, and one per qubit at the end.
+Fusion (HGM_PLAN_FUSE) and row scaling (HGM_PLAN_PIVOT) keep the unitary too."""
 import ctypes as C
 
 import numpy as np
@@ -141,3 +142,110 @@ def test_fuse_flag_expands_iterations():
     plan = hq.Plan(n, ops, tun, iterations=5)
     total = sum(len(idx) for _, idx in plan.passes())
     assert total < len(ops) * 5
+
+
+def pivot(n, ops):
+    cnt = C.c_int64()
+    N.check(N.lib.hgm_pivot_rows(n, ops, len(ops), None, 0, C.byref(cnt)))
+    out = (N.Op * max(1, cnt.value))()
+    N.check(N.lib.hgm_pivot_rows(n, ops, len(ops), out, cnt.value, C.byref(cnt)))
+    return out if cnt.value else (N.Op * 0)()
+
+
+def apply_list(n, ops, psi, dtype):
+    """The gate list applied gate by gate in numpy at `dtype` (a check of the
+    list's numerics, independent of the oracle)."""
+    psi = psi.astype(dtype).copy()
+    idx = np.arange(1 << n)
+    for i in range(len(ops)):
+        o = ops[i]
+        m = (np.array(o.m[0::2]) + 1j * np.array(o.m[1::2])).astype(dtype)
+        b = 1 << o.target_bit
+        sel = ((idx & b) == 0) & ((idx & o.control_mask) == o.control_value)
+        i0 = idx[sel]
+        x, y = psi[i0].copy(), psi[i0 | b].copy()
+        psi[i0] = m[0] * x + m[1] * y
+        psi[i0 | b] = m[2] * x + m[3] * y
+    return psi
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_pivoted_list_is_the_same_unitary(seed):
+    """Row scaling (HGM_PLAN_PIVOT) keeps the unitary, global phase included;
+    every uncontrolled dense gate it emits has an exact 1 in each row and an
+    other entry of magnitude <= 1."""
+    n = 7
+    rng = np.random.default_rng(200 + seed)
+    t, cm, cv, mats = _random_gate_list(rng, n, 120)
+    for k in range(0, len(cm), 2):
+        cm[k] = cv[k] = 0
+    ops = hq.make_ops(t, cm, cv, mats)
+    f = pivot(n, ops)
+    assert len(f) <= 2 * len(ops) + n + 1
+    dense = 0
+    for i in range(len(f)):
+        m = np.array(f[i].m[0::2]) + 1j * np.array(f[i].m[1::2])
+        if f[i].control_mask or (m[1] == 0 and m[2] == 0) or (m[0] == 0 and m[3] == 0):
+            continue
+        dense += 1
+        for row in (m[:2], m[2:]):
+            assert 1 in (row[0], row[1])
+            assert min(abs(row[0]), abs(row[1])) <= 1 + 1e-15
+    assert dense >= 10
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+    want = run(n, ops, psi0)
+    assert np.abs(run(n, f, psi0) - want).max() < 1e-13
+    ff = pivot(n, fuse(n, ops))           # fuse then scale rows, as the planner does
+    assert np.abs(run(n, ff, psi0) - want).max() < 1e-13
+
+
+def test_pivot_turns_x_into_a_swap_and_absorbs_diagonals():
+    """An uncontrolled anti-diagonal gate becomes the exact X (a register swap),
+    an uncontrolled diagonal gate disappears into the carried factors, and the
+    carried factors come out at the end as diagonal gates."""
+    n = 3
+    y = np.array([[0, -1j], [1j, 0]])
+    s = np.diag([1, 1j])
+    ops = hq.make_ops([0, 1, 2], [0, 0, 0], [0, 0, 0], [y, s, y @ s])
+    f = pivot(n, ops)
+    mats = [np.array(f[i].m[0::2]) + 1j * np.array(f[i].m[1::2]) for i in range(len(f))]
+    assert np.array_equal(mats[0], [0, 1, 1, 0]) and f[0].target_bit == 0
+    assert all(m[1] == 0 and m[2] == 0 for m in mats[2:])       # the rest are diagonal
+    psi0 = np.arange(1 << n) + 1j
+    assert np.abs(run(n, f, psi0) - run(n, ops, psi0)).max() < 1e-13
+
+
+def test_pivot_rescales_deep_circuits_in_fp32():
+    """2000 Hadamard gates: every one scales the stored amplitudes by sqrt(2),
+    so the list must carry uniform rescales; applied in complex64 it stays
+    finite and equals the original within FP32 rounding."""
+    n = 5
+    h = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+    rng = np.random.default_rng(4)
+    targets = list(rng.integers(0, n, 2000))
+    ops = hq.make_ops(targets, [0] * 2000, [0] * 2000, [h] * 2000)
+    f = pivot(n, ops)
+    scales = [i for i in range(len(f)) if f[i].m[0] == f[i].m[6] and f[i].m[2] == f[i].m[4] == 0 and
+              f[i].m[1] == 0 and f[i].m[0] != 1]
+    assert len(scales) >= 2000 * 0.5 / 24 - 1
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+    got = apply_list(n, f, psi0, np.complex64)
+    assert np.all(np.isfinite(got))
+    want = apply_list(n, ops, psi0, np.complex128)
+    assert np.abs(got - want).max() < 1e-4
+
+
+def test_pivot_flag_plans_with_fusion():
+    n = 10
+    d = hq.DGM()
+    d.setFunction(hq.RandomLayers(n, 4, 5, d))
+    ops = d.ops()
+    flags = N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT
+    tun = N.default_tuning(block_size=64, repeat_count=4, gpu_region_bits=9, gpu_coalesced_bits=2,
+                           planner=flags, kernel=N.HGM_KERNEL_REG)
+    assert len(hq.Plan(n, ops, tun, iterations=2)) >= 1
+    bad = hq.make_ops([n], [0], [0], [np.eye(2)])
+    with pytest.raises(N.HGMError):
+        pivot(n, bad)

@@ -13,7 +13,8 @@ from tests.test_oracle import _random_gate_list, oracle_run_gates
 pytestmark = pytest.mark.gpu
 
 FAMILIES = [N.HGM_KERNEL_LDS, N.HGM_KERNEL_REG, N.HGM_KERNEL_JIT]
-PLANNERS = [N.HGM_PLAN_GREEDY, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_FOLD_PHASES]
+PLANNERS = [N.HGM_PLAN_GREEDY, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_FOLD_PHASES,
+            N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT]
 
 
 def say(*a):
@@ -83,6 +84,18 @@ def test_msb_targets_and_controls_every_family():
             hq.Plan(n, ops, tun).run(s)
             say("msb", kern, planner)
             assert np.abs(s.download() - want).max() < 1e-12, (kern, planner)
+
+
+def test_state_of_2_pow_32_amplitudes():
+    """32 qubits in FP32 (32 GiB): the clear, basis and readout launches stay
+    within a dispatch's 2^32 - 1 work-items (grid-stride loops)."""
+    say("state 32q fp32")
+    s = hq.State(32, np.complex64)
+    s.set_basis((1 << 32) - 5)
+    idx, p = s.argmax()
+    assert idx == (1 << 32) - 5 and p == 1.0
+    assert abs(s.norm2() - 1) < 1e-6
+    del s
 
 
 def test_allocations_beyond_hbm_fail_cleanly():

@@ -87,7 +87,8 @@ def test_random_gates_tuning_sweep(n):
     want = oracle_run_gates(n, t, cm, cv, mats, state=psi0)
     for B, R, r, c in TUNINGS:
         for planner in (N.HGM_PLAN_GREEDY, N.HGM_PLAN_WINDOW,
-                        N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_FOLD_PHASES):
+                        N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_FOLD_PHASES,
+                        N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT):
             for ops_block in (300, 5):
                 say("sweep", n, (B, R, r, c), planner, ops_block)
                 tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c,
@@ -136,7 +137,8 @@ def test_grover_22q_config():
     ops = d.ops()
     d.setFunction(hq.Hadamard(n, 0, n))
     hops = d.ops()
-    for planner in (N.HGM_PLAN_GREEDY, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_FOLD_PHASES):
+    for planner in (N.HGM_PLAN_GREEDY, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_FOLD_PHASES,
+                    N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT):
         tun = N.default_tuning(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=4,
                                planner=planner)
         s = hq.State(n)
@@ -189,7 +191,10 @@ def test_random_layers_20q_vs_oracle():
     for tun in (N.default_tuning(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4),
                 N.default_tuning(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=2),
                 N.default_tuning(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
-                                 kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FOLD_PHASES)):
+                                 kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FOLD_PHASES),
+                N.default_tuning(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
+                                 kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE |
+                                 N.HGM_PLAN_PIVOT)):
         s = hq.State(n)
         s.set_basis(0)
         st = s.apply(ops, tun)
@@ -197,14 +202,21 @@ def test_random_layers_20q_vs_oracle():
         assert np.abs(s.download() - want).max() < TOL64
 
 
-@pytest.mark.parametrize("n,dtype,tol", [(30, np.complex128, 1e-12), (28, np.complex64, 2e-5)])
-def test_random_layers_full_size_round_trip(n, dtype, tol):
+PIVOT_JIT = dict(kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT)
+
+
+@pytest.mark.parametrize("n,dtype,tol,tuning", [
+    (30, np.complex128, 1e-12, (256, 16, 13, 2, {})),
+    (28, np.complex64, 2e-5, (256, 16, 13, 2, {})),
+    (28, np.complex64, 2e-5, (512, 8, 13, 4, PIVOT_JIT))])
+def test_random_layers_full_size_round_trip(n, dtype, tol, tuning):
     """U then U^dagger returns |0>; the norm stays 1 (size-independent checks at full size)."""
     depth = 20
     d = hq.DGM()
     d.setFunction(hq.RandomLayers(n, depth, 1, d))
     ops = d.ops()
-    tun = N.default_tuning(block_size=256, repeat_count=16, gpu_region_bits=13, gpu_coalesced_bits=2)
+    B, R, r, c, extra = tuning
+    tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c, **extra)
     s = hq.State(n, dtype)
     s.set_basis(0)
     st = s.apply(ops, tun)
@@ -306,7 +318,9 @@ def test_jit_kernel_parity(dtype, tol):
     want = oracle_run_gates(n, t, cm, cv, mats, state=psi0)
     for B, R, r, c, planner in [(64, 8, 10, 3, N.HGM_PLAN_GREEDY), (32, 4, 8, 2, N.HGM_PLAN_GREEDY),
                                 (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FOLD_PHASES),
-                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_FOLD_PHASES)]:
+                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_FOLD_PHASES),
+                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_PIVOT),
+                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT)]:
         tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c,
                                kernel=N.HGM_KERNEL_JIT, ops_block=60, planner=planner)
         plan = hq.Plan(n, ops, tun, 1, dtype)
@@ -318,13 +332,14 @@ def test_jit_kernel_parity(dtype, tol):
         assert np.abs(s.download() - want).max() < tol
 
 
-def test_jit_random_layers_30q_round_trip():
+@pytest.mark.parametrize("flag", [N.HGM_PLAN_FOLD_PHASES, N.HGM_PLAN_PIVOT])
+def test_jit_random_layers_30q_round_trip(flag):
     n = 30
     d = hq.DGM()
     d.setFunction(hq.RandomLayers(n, 20, 1, d))
     ops = d.ops()
     tun = N.default_tuning(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
-                           kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FOLD_PHASES)
+                           kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | flag)
     s = hq.State(n)
     s.set_basis(0)
     st = hq.Plan(n, ops, tun).run(s)

@@ -283,6 +283,30 @@ def test_header_is_plain_c_and_links(tmp_path):
     assert out[0] == "1" and out[1] == "hgm-mi355x"
 
 
+def test_register_kernels_keep_amplitudes_in_registers():
+    """The interpreted register kernels (per-pass and batched) compile with no
+    scratch memory: the amplitude array lives in VGPRs. A runtime index into it
+    (for instance from merged switch cases) would move it to scratch and cost
+    several times the kernel time."""
+    import shutil
+    import subprocess
+    if not shutil.which("hipcc") and not os.path.exists("/opt/rocm/bin/hipcc"):
+        pytest.skip("no hipcc")
+    out = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "simulador_quantico_amd", "csrc"), "resource-usage"],
+                         check=True, capture_output=True, text=True, timeout=600)
+    usage, name = {}, None
+    for line in (out.stdout + out.stderr).splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and name:
+            usage[name] = int(m.group(1))
+    reg = {k: v for k, v in usage.items() if "k_pass_reg" in k or "k_batch_reg" in k}
+    assert len(reg) == 16, sorted(usage)
+    assert all(v == 0 for v in reg.values()), reg
+
+
 def test_gpu_entry_points_fail_cleanly_without_a_device():
     """Without a visible device the serving and autotune entry points return an error, not a crash."""
     if N.device_count() > 0:

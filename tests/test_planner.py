@@ -142,3 +142,32 @@ def test_jit_source_compiles_and_dedupes():
     kernels = src.count("__global__")
     assert plan.kernel == N.HGM_KERNEL_JIT and 0 < kernels < len(plan)
     assert "v_pk" not in src and "hgm_pass_0" in src
+
+
+def test_jit_source_of_pivoted_plan_drops_a_third_of_the_multiplies():
+    """Folded dense updates take 3 products per output component (12 per pair);
+    with rows scaled to an exact 1 (HGM_PLAN_PIVOT) the generated code has 2
+    (8 per pair). Both plans compile for gfx950."""
+    n = 14
+    d = hq.DGM()
+    d.setFunction(hq.RandomLayers(n, 6, 11, d))
+    ops = d.ops()
+    counts = {}
+    for name, flag in (("fold", N.HGM_PLAN_FOLD_PHASES), ("pivot", N.HGM_PLAN_PIVOT)):
+        tun = N.default_tuning(block_size=64, repeat_count=4, gpu_region_bits=9, gpu_coalesced_bits=3,
+                               kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | flag)
+        plan = hq.Plan(n, ops, tun)
+        counts[name] = plan.jit_source().count(" * ")
+    assert counts["pivot"] < 0.72 * counts["fold"]
+
+
+def test_jit_dispatch_limit():
+    """A plan-compiled pass launches one workgroup per tile; a launch holds at
+    most 2^32 - 1 work-items, so 33 qubits need repeat_count >= 2."""
+    n = 33
+    ops = hq.make_ops([0], [0], [0], [np.array([[0, 1], [1, 0]])])
+    tun = N.default_tuning(block_size=1024, repeat_count=1, gpu_region_bits=11, gpu_coalesced_bits=3,
+                           kernel=N.HGM_KERNEL_JIT)
+    with pytest.raises(N.HGMError) as e:
+        hq.Plan(n, ops, tun, 1, np.complex64)
+    assert "dispatch" in str(e.value)

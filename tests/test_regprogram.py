@@ -5,7 +5,8 @@ The model follows k_pass_reg: thread t holds A amplitudes; element (t, k)
 sits at tile index thr[0][t & 31] | thr[1][t >> 5] | reg[k] of the current
 phase map; R_REMAP moves the tile to the next map; R_DENSE / R_ANTI update
 register pairs differing in register bit `rbit`; R_PHASE multiplies where the
-register, thread and global conditions hold."""
+register, thread and global conditions hold; R_DENSE_PIV* apply rows with an
+exact unit entry, out_i = x[p_i] + t_i x[1 - p_i], with the pivots in tval."""
 import numpy as np
 import pytest
 
@@ -13,7 +14,7 @@ import simulador_quantico_amd as hq
 from simulador_quantico_amd import _native as N
 from tests.test_oracle import _random_gate_list, oracle_run_gates
 
-R_DENSE, R_ANTI, R_PHASE, R_REMAP, R_SWAP, R_PHASE_REAL, R_DENSE_REAL = 0, 1, 2, 3, 4, 5, 6
+R_DENSE, R_ANTI, R_PHASE, R_REMAP, R_SWAP, R_PHASE_REAL, R_DENSE_REAL, R_DENSE_PIV, R_DENSE_PIV_REAL = range(9)
 
 
 def pdep(v, mask):
@@ -68,8 +69,22 @@ def emulate(n, plan, psi):
                     continue
                 if (base & int(o["gmask"])) != int(o["gval"]):
                     continue
-                lane = (t & int(o["tmask"])) == int(o["tval"])
                 m = o["m"][0::2] + 1j * o["m"][1::2]
+                if kind in (R_DENSE_PIV, R_DENSE_PIV_REAL):
+                    assert int(o["tmask"]) == 0 and int(o["rmask"]) == 0
+                    if kind == R_DENSE_PIV_REAL:
+                        assert o["m"][1] == 0 and o["m"][3] == 0
+                    p0, p1 = int(o["tval"]) & 1, int(o["tval"]) >> 1
+                    bit = int(o["rbit"])
+                    for k0 in range(A):
+                        if (k0 >> bit) & 1:
+                            continue
+                        k1 = k0 | (1 << bit)
+                        xy = (a[:, k0].copy(), a[:, k1].copy())
+                        a[:, k0] = xy[p0] + m[0] * xy[1 - p0]
+                        a[:, k1] = xy[p1] + m[1] * xy[1 - p1]
+                    continue
+                lane = (t & int(o["tmask"])) == int(o["tval"])
                 if kind in (R_PHASE, R_PHASE_REAL):
                     for k in range(A):
                         if (k & int(o["rmask"])) == int(o["rval"]):
@@ -92,21 +107,48 @@ def emulate(n, plan, psi):
     return psi
 
 
-@pytest.mark.parametrize("n,B,R,r,c", [(7, 4, 2, 4, 1), (9, 8, 4, 6, 2), (10, 16, 4, 7, 2), (10, 4, 8, 6, 0),
-                                       (8, 2, 16, 6, 3), (6, 32, 1, 6, 2), (11, 32, 8, 9, 4)])
-def test_register_program_matches_oracle(n, B, R, r, c):
+SHAPES = [(7, 4, 2, 4, 1), (9, 8, 4, 6, 2), (10, 16, 4, 7, 2), (10, 4, 8, 6, 0), (8, 2, 16, 6, 3), (6, 32, 1, 6, 2),
+          (11, 32, 8, 9, 4)]
+
+
+@pytest.mark.parametrize("planner", [N.HGM_PLAN_GREEDY, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT])
+@pytest.mark.parametrize("n,B,R,r,c", SHAPES)
+def test_register_program_matches_oracle(n, B, R, r, c, planner):
     rng = np.random.default_rng(n * 7 + r)
     t, cm, cv, mats = _random_gate_list(rng, n, 80)
     ops = hq.make_ops(t, cm, cv, mats)
     tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c,
-                           kernel=N.HGM_KERNEL_REG, ops_block=40)
+                           kernel=N.HGM_KERNEL_REG, ops_block=40, planner=planner)
     plan = hq.Plan(n, ops, tun)
     assert plan.kernel == N.HGM_KERNEL_REG
+    kinds = np.concatenate([plan.program(i)[0]["kind"] for i in range(len(plan))])
+    if planner & N.HGM_PLAN_PIVOT:
+        assert np.any(kinds == R_DENSE_PIV)
     psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
     psi0 /= np.linalg.norm(psi0)
     want = oracle_run_gates(n, t, cm, cv, mats, state=psi0)
     got = emulate(n, plan, psi0)
     assert np.abs(got - want).max() < 1e-12
+
+
+def test_pivot_program_of_grover_is_real():
+    """Fused Grover gates are real, and stay real under row scaling: every
+    dense update of the program is R_DENSE_PIV_REAL (4 multiply-adds per pair)."""
+    n, k = 9, 5
+    d = hq.DGM()
+    d.setFunction(hq.GroverIteration(n, 300))
+    ops = d.ops()
+    tun = N.default_tuning(block_size=16, repeat_count=4, gpu_region_bits=7, gpu_coalesced_bits=2,
+                           kernel=N.HGM_KERNEL_REG, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT)
+    plan = hq.Plan(n, ops, tun, iterations=k)
+    kinds = np.concatenate([plan.program(i)[0]["kind"] for i in range(len(plan))])
+    assert np.any(kinds == R_DENSE_PIV_REAL)
+    assert not np.any(np.isin(kinds, (R_DENSE, R_DENSE_REAL, R_DENSE_PIV)))
+    psi0 = np.full(1 << n, 2 ** (-n / 2), complex)
+    t, cm, cv, mats = hq.ops_to_arrays(ops)
+    want = oracle_run_gates(n, np.tile(t, k), np.tile(cm, k), np.tile(cv, k), list(mats) * k, state=psi0)
+    assert abs(abs(want[300]) ** 2 - np.sin(11 * np.arcsin(2 ** (-n / 2))) ** 2) < 1e-12
+    assert np.abs(emulate(n, plan, psi0) - want).max() < 1e-12
 
 
 def test_register_program_keeps_loads_coalesced():
