@@ -56,13 +56,14 @@ def algorithmic_flops(ops, n: int, iterations: int) -> float:
 METRIC = "amp-updates/s + %HBM roofline @30q; circuits/s and p99 latency on serving mix"
 
 WORKLOADS = {
-    # name: (qubits, dtype, builder, tuning); planner 96 = greedy + gate fusion + row scaling
+    # name: (qubits, dtype, builder, tuning); planner 96 = greedy + gate fusion + row scaling,
+    # 224 = the same + pass balancing
     "random30": dict(qubits=30, dtype="c128", kind="random", depth=20, seed=1,
                      tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=3,
-                                 kernel=3, planner=96)),
+                                 kernel=3, planner=224)),
     "u3layers30": dict(qubits=30, dtype="c128", kind="u3layers", depth=20, seed=1,
                        tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=3,
-                                   kernel=3, planner=96)),
+                                   kernel=3, planner=224)),
     "random32f": dict(qubits=32, dtype="c64", kind="random", depth=20, seed=1,
                       tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=4,
                                   kernel=3, planner=96)),
@@ -86,7 +87,7 @@ WORKLOADS = {
 
 def planner_name(flags: int) -> str:
     base = "window" if (flags & 15) == 1 else "greedy"
-    extra = [n for bit, n in ((32, "gate fusion"), (64, "row scaling"), (16, "phase folding"))
+    extra = [n for bit, n in ((32, "gate fusion"), (64, "row scaling"), (16, "phase folding"), (128, "pass balancing"))
              if flags & bit and not (bit == 16 and flags & 64)]
     return " + ".join([base] + extra)
 
@@ -94,12 +95,13 @@ def planner_name(flags: int) -> str:
 KERNEL_NAMES = {1: "k_pass_lds", 2: "k_pass_reg", 3: "hgm_pass_* (plan-compiled)"}
 
 
-def measured_traffic(workload: str, tun: dict):
-    """HBM bytes per pass-kernel launch measured by the FETCH_SIZE / WRITE_SIZE
-    counters (rocprofv3 --pmc, one run each, scaled by the same counters on a
-    plain copy of known size), stored beside the profile they come from; None
-    when no measurement exists for this workload and tile shape."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+def measured_traffic(workload: str, tun: dict, kind: str = "traffic"):
+    """A counter measurement stored beside the profile it comes from, or None
+    when none exists for this workload and tile shape. kind "traffic": HBM
+    bytes per pass-kernel launch from the FETCH_SIZE / WRITE_SIZE counters
+    (rocprofv3 --pmc, one run each, scaled by the same counters on a plain copy
+    of known size); kind "valu": VALU and FP64 wave instructions of one step."""
+    path = os.path.join(ROOT, "profiles", f"{kind}_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -544,15 +546,24 @@ def main():
                      "kernel": KERNEL_NAMES.get(plan.kernel, "?"),
                      "bytes_per_launch": bytes_per_launch,
                      "mean_launch_ms": mean_launch_ms,
-                     "compute": {"achieved": tflops, "peak": fpeak, "unit": "TFLOP/s", "frac": tflops / fpeak,
-                                 "flops_per_step": flops,
-                                 "counting": "dense gate 14 flop/amplitude, (anti)diagonal 6 flop per changed "
-                                             "amplitude, over all passes' kernel time"}},
+                     "compute": {"achieved": None, "peak": fpeak, "unit": "TFLOP/s", "frac": None,
+                                 "executed_flops_per_step": None,
+                                 "reference_formula": {
+                                     "tflops": tflops, "flops_per_step": flops,
+                                     "counting": "dense gate 14 flop/amplitude, (anti)diagonal 6 flop per changed "
+                                                 "amplitude (the reference kernel's arithmetic), over all passes' "
+                                                 "kernel time"}}},
     }
     traffic = measured_traffic(args.workload, tun)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic["traffic_bytes_per_launch"]
         out["roofline"]["traffic_source"] = traffic["source"]
+    valu = measured_traffic(args.workload, tun, "valu")
+    if valu is not None:   # executed FP64 work (counters) over this run's kernel time
+        executed = valu["executed_fp64_flops_per_step"] * args.steps / (kernel_ms * 1e-3) / 1e12
+        out["roofline"]["compute"].update(achieved=executed, frac=executed / fpeak,
+                                          executed_flops_per_step=valu["executed_fp64_flops_per_step"],
+                                          executed_source=valu["source"])
     default_run = args.tuning is None and args.kernel is None and args.planner is None and args.ops_block is None
     if not args.no_cpu_baseline:
         try:

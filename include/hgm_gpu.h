@@ -74,7 +74,11 @@ enum { HGM_PLAN_GREEDY = 0, /* dependency-aware lookahead, any set of tile-local
         * [2^-24, 2^24], and at the end). Subsumes HGM_PLAN_FOLD_PHASES. Same unitary; results
         * differ by rounding only (a dense update drops to 8 multiply-adds per pair, 4 when the
         * scaled matrix is real). */
-       HGM_PLAN_PIVOT = 64 };
+       HGM_PLAN_PIVOT = 64,
+       /* flag (greedy planner): after packing, move gates between neighbouring passes from the one
+        * with more arithmetic to the one with less, where the gate commutes with the gates it passes
+        * and its target is tile-local there; the pass count is unchanged. */
+       HGM_PLAN_BALANCE = 128 };
 
 /* kernel families */
 enum { HGM_KERNEL_AUTO = 0,  /* the register kernel when the tile shape allows it, else LDS */

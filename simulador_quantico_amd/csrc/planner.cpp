@@ -76,6 +76,97 @@ void finish_pass(uint64_t local, const Plan &plan, Pass &pass) {
         if ((local >> b) & 1) pass.local_pos[j++] = (int8_t)b;
 }
 
+// Arithmetic of one gate per amplitude pair as the generated pass code does
+// it: per output component, a term with coefficient +-1 is an addend and
+// every other nonzero term one multiply or FMA; controls leave 2^-k of the
+// pairs to update. Used to balance passes, not as a flop count.
+double gate_cost(const hgm_op &o) {
+    auto unit = [](double re, double im) { return im == 0.0 && (re == 1.0 || re == -1.0); };
+    double ops = 0;
+    for (int row = 0; row < 2; ++row) {
+        const double *a = o.m + 4 * row, *b = o.m + 4 * row + 2;
+        // both components of the row's output use the same coefficients
+        int terms = 0, units = 0;
+        for (const double *e : {a, b}) {
+            const int nz = (e[0] != 0.0) + (e[1] != 0.0);
+            if (unit(e[0], e[1])) { ++terms; ++units; }
+            else terms += nz;
+        }
+        if (terms == 1 && units == 1) continue;                  // a copy or a sign
+        ops += 2.0 * (terms - (units ? 1 : 0));
+    }
+    return ops / double(uint64_t(1) << __builtin_popcountll(o.control_mask));
+}
+
+// Moves gates between neighbouring passes, from the one with more arithmetic
+// to the one with less, while that lowers the larger of the two: a gate
+// moves to the front of the next pass when it commutes with every gate after
+// it in its own pass, or to the end of the previous pass when it commutes
+// with every gate before it, and only where its target is tile-local. The
+// pass count and the set of gates are unchanged.
+void balance_passes(const std::vector<GateInfo> &info, const hgm_op *ops, int64_t n_ops, int64_t cap,
+                    const std::vector<uint64_t> &local, std::vector<std::vector<int64_t>> &lists) {
+    const size_t P = lists.size();
+    if (P < 2) return;
+    std::vector<double> cost(P, 0.0);
+    for (size_t i = 0; i < P; ++i)
+        for (int64_t k : lists[i]) cost[i] += gate_cost(ops[k % n_ops]);
+    for (int sweep = 0; sweep < 8; ++sweep) {
+        bool moved = false;
+        for (size_t i = 0; i + 1 < P; ++i) {
+            // forward: the tail of pass i to the front of pass i+1
+            while (cost[i] > cost[i + 1] && (int64_t)lists[i + 1].size() < cap) {
+                const double gap = cost[i] - cost[i + 1];
+                uint64_t bw = 0, br = 0;
+                int64_t pick = -1;
+                for (int64_t j = (int64_t)lists[i].size() - 1; j >= 0; --j) {
+                    const GateInfo &g = info[lists[i][j] % n_ops];
+                    const double c = gate_cost(ops[lists[i][j] % n_ops]);
+                    if (!(g.writes & (bw | br)) && !(g.reads & bw) && !(g.need & ~local[i + 1]) && c > 0 && c < gap) {
+                        pick = j;
+                        break;
+                    }
+                    bw |= g.writes;
+                    br |= g.reads;
+                }
+                if (pick < 0) break;
+                const int64_t k = lists[i][pick];
+                const double c = gate_cost(ops[k % n_ops]);
+                lists[i].erase(lists[i].begin() + pick);
+                lists[i + 1].insert(lists[i + 1].begin(), k);
+                cost[i] -= c;
+                cost[i + 1] += c;
+                moved = true;
+            }
+            // backward: the head of pass i+1 to the end of pass i
+            while (cost[i + 1] > cost[i] && (int64_t)lists[i].size() < cap) {
+                const double gap = cost[i + 1] - cost[i];
+                uint64_t bw = 0, br = 0;
+                int64_t pick = -1;
+                for (int64_t j = 0; j < (int64_t)lists[i + 1].size(); ++j) {
+                    const GateInfo &g = info[lists[i + 1][j] % n_ops];
+                    const double c = gate_cost(ops[lists[i + 1][j] % n_ops]);
+                    if (!(g.writes & (bw | br)) && !(g.reads & bw) && !(g.need & ~local[i]) && c > 0 && c < gap) {
+                        pick = j;
+                        break;
+                    }
+                    bw |= g.writes;
+                    br |= g.reads;
+                }
+                if (pick < 0) break;
+                const int64_t k = lists[i + 1][pick];
+                const double c = gate_cost(ops[k % n_ops]);
+                lists[i + 1].erase(lists[i + 1].begin() + pick);
+                lists[i].push_back(k);
+                cost[i + 1] -= c;
+                cost[i] += c;
+                moved = true;
+            }
+        }
+        if (!moved) break;
+    }
+}
+
 }  // namespace
 
 int classify(const double m[8]) {
@@ -114,7 +205,7 @@ int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::
     if (B > 1024) { *err = "Erro de tuning: block_size must be <= 1024 on gfx950"; return HGM_E_TUNING; }
     if (!is_pow2(B) || !is_pow2(R)) { *err = "Erro de tuning: block_size and repeat_count must be powers of two"; return HGM_E_TUNING; }
     if (t.ops_block < 1) { *err = "Erro de tuning: ops_block must be >= 1"; return HGM_E_TUNING; }
-    const int base_planner = t.planner & ~(HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE | HGM_PLAN_PIVOT);
+    const int base_planner = t.planner & ~(HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE | HGM_PLAN_PIVOT | HGM_PLAN_BALANCE);
     if (base_planner != HGM_PLAN_GREEDY && base_planner != HGM_PLAN_WINDOW) {
         *err = "unknown planner";
         return HGM_E_TUNING;
@@ -485,6 +576,9 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
             *local_out = local;
         };
         const int e = r - c;
+        std::vector<std::vector<int64_t>> lists;
+        std::vector<Pass> shells;
+        std::vector<uint64_t> locals;
         while (head < N) {
             uint64_t local;
             scan(0, true, &local, chosen);
@@ -498,8 +592,14 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
             while (head < N && done[head]) ++head;
             Pass p;
             finish_pass(local, plan, p);
-            fill_pass_ops(info, ops, n_ops, chosen, p, plan);
-            plan.passes.push_back(p);
+            shells.push_back(p);
+            locals.push_back(p.local_mask);
+            lists.push_back(chosen);
+        }
+        if (t.planner & HGM_PLAN_BALANCE) balance_passes(info, ops, n_ops, cap, locals, lists);
+        for (size_t i = 0; i < shells.size(); ++i) {
+            fill_pass_ops(info, ops, n_ops, lists[i], shells[i], plan);
+            plan.passes.push_back(shells[i]);
         }
     }
     const int A = 2 * plan.shape.R;

@@ -171,3 +171,31 @@ def test_jit_dispatch_limit():
     with pytest.raises(N.HGMError) as e:
         hq.Plan(n, ops, tun, 1, np.complex64)
     assert "dispatch" in str(e.value)
+
+
+def test_balance_keeps_passes_and_gates():
+    """HGM_PLAN_BALANCE moves gates between neighbouring passes only: the pass
+    count and the multiset of gates stay, the heaviest pass gets no heavier,
+    and every gate's target is local in the pass it lands in."""
+    n = 24
+    d = hq.DGM()
+    d.setFunction(hq.RandomLayers(n, 12, 3, d))
+    ops = d.ops()
+    base = N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT
+    plans = {}
+    for flags in (base, base | N.HGM_PLAN_BALANCE):
+        tun = N.default_tuning(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=3,
+                               kernel=N.HGM_KERNEL_REG, planner=flags)
+        plans[flags] = hq.Plan(n, ops, tun)
+    a, b = plans[base], plans[base | N.HGM_PLAN_BALANCE]
+    assert len(a) == len(b)
+    ga = sorted(i for _, idx in a.passes() for i in idx)
+    gb = sorted(i for _, idx in b.passes() for i in idx)
+    assert ga == gb
+    from tests.test_fold import fuse, pivot
+    t, cm, cv, mats = hq.ops_to_arrays(pivot(n, fuse(n, ops)))   # the list the planner schedules
+    assert ga == list(range(len(t)))
+    for mask, idx in b.passes():
+        for i in idx:
+            if not (mats[i][1] == 0 and mats[i][2] == 0):
+                assert (mask >> int(t[i])) & 1
