@@ -385,7 +385,7 @@ def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026,
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="random30", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -508,10 +508,12 @@ def main():
         plan.run(state)
     kernel_ms = 0.0
     passes = 0
+    step_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         st = plan.run(state)
         kernel_ms += st["kernel_ms"]
+        step_ms.append(st["kernel_ms"])
         passes += st["passes"]
     wall = time.perf_counter() - t0
     gates = len(ops) * iterations
@@ -534,6 +536,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps,
+        "kernel_ms_per_step_median": float(np.median(step_ms)),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
