@@ -199,15 +199,23 @@ __device__ __forceinline__ void reg_dense_real(typename Vec<T>::type (&a)[A], co
     }
 }
 
-// dispatch on the runtime register bit (tile-uniform) to unrolled code
-#define HGM_REG_BITS(CALL)                                  \
-    switch (bit) {                                          \
-        case 0: CALL(0); break;                             \
-        case 1: if constexpr (A > 2) { CALL(1); } break;    \
-        case 2: if constexpr (A > 4) { CALL(2); } break;    \
-        case 3: if constexpr (A > 8) { CALL(3); } break;    \
-        case 4: if constexpr (A > 16) { CALL(4); } break;   \
-        default: break;                                     \
+// dispatch on the runtime register bit (tile-uniform) to unrolled code. With
+// STRICT, the cases no bit can reach (the planner emits register bits below
+// log2(A) only) are marked unreachable: every remaining case then updates the
+// amplitudes in place, and the compiler stops copying them to a second
+// register set per gate, for a few more VGPRs (used by the batched kernel,
+// whose register budget allows it)
+#define HGM_NO_CASE                               \
+    if constexpr (STRICT) __builtin_unreachable(); \
+    break;
+#define HGM_REG_BITS(CALL)                                             \
+    switch (bit) {                                                     \
+        case 0: CALL(0); break;                                        \
+        case 1: if constexpr (A > 2) { CALL(1); break; } HGM_NO_CASE   \
+        case 2: if constexpr (A > 4) { CALL(2); break; } HGM_NO_CASE   \
+        case 3: if constexpr (A > 8) { CALL(3); break; } HGM_NO_CASE   \
+        case 4: if constexpr (A > 16) { CALL(4); break; } HGM_NO_CASE  \
+        default: HGM_NO_CASE                                           \
     }
 
 // Rows with an exact unit entry (R_DENSE_PIV*): out_i = x[P_i] + t_i * x[1 - P_i],
@@ -237,7 +245,7 @@ __device__ __forceinline__ void reg_dense_piv(typename Vec<T>::type (&a)[A], T t
     }
 }
 
-template <class T, int A, int P0, int P1, bool REAL>
+template <class T, int A, int P0, int P1, bool REAL, bool STRICT>
 __device__ __forceinline__ void reg_dense_piv_bits(typename Vec<T>::type (&a)[A], int bit, T t0r, T t0i, T t1r,
                                                    T t1i) {
 #define HGM_CALL(B) reg_dense_piv<T, A, B, P0, P1, REAL>(a, t0r, t0i, t1r, t1i)
@@ -245,14 +253,14 @@ __device__ __forceinline__ void reg_dense_piv_bits(typename Vec<T>::type (&a)[A]
 #undef HGM_CALL
 }
 
-template <class T, int A, bool REAL>
+template <class T, int A, bool REAL, bool STRICT>
 __device__ __forceinline__ void reg_dense_piv_any(typename Vec<T>::type (&a)[A], const RegOp &o) {
     const T t0r = (T)o.m[0], t0i = (T)o.m[1], t1r = (T)o.m[2], t1i = (T)o.m[3];
     switch (o.tval & 3u) {
-        case 0: reg_dense_piv_bits<T, A, 0, 0, REAL>(a, o.rbit, t0r, t0i, t1r, t1i); break;
-        case 1: reg_dense_piv_bits<T, A, 1, 0, REAL>(a, o.rbit, t0r, t0i, t1r, t1i); break;
-        case 2: reg_dense_piv_bits<T, A, 0, 1, REAL>(a, o.rbit, t0r, t0i, t1r, t1i); break;
-        default: reg_dense_piv_bits<T, A, 1, 1, REAL>(a, o.rbit, t0r, t0i, t1r, t1i); break;
+        case 0: reg_dense_piv_bits<T, A, 0, 0, REAL, STRICT>(a, o.rbit, t0r, t0i, t1r, t1i); break;
+        case 1: reg_dense_piv_bits<T, A, 1, 0, REAL, STRICT>(a, o.rbit, t0r, t0i, t1r, t1i); break;
+        case 2: reg_dense_piv_bits<T, A, 0, 1, REAL, STRICT>(a, o.rbit, t0r, t0i, t1r, t1i); break;
+        default: reg_dense_piv_bits<T, A, 1, 1, REAL, STRICT>(a, o.rbit, t0r, t0i, t1r, t1i); break;
     }
 }
 
@@ -282,11 +290,11 @@ __device__ __forceinline__ void reg_swap(typename Vec<T>::type (&a)[A]) {
     }
 }
 
-template <class T, int A>
+template <class T, int A, bool STRICT>
 __device__ __forceinline__ void reg_gate(typename Vec<T>::type (&a)[A], const RegOp &o, int t) {
     const int bit = o.rbit;
-    if (o.kind == R_DENSE_PIV) { reg_dense_piv_any<T, A, false>(a, o); return; }
-    if (o.kind == R_DENSE_PIV_REAL) { reg_dense_piv_any<T, A, true>(a, o); return; }
+    if (o.kind == R_DENSE_PIV) { reg_dense_piv_any<T, A, false, STRICT>(a, o); return; }
+    if (o.kind == R_DENSE_PIV_REAL) { reg_dense_piv_any<T, A, true, STRICT>(a, o); return; }
     if (o.kind == R_SWAP) {
 #define HGM_CALL(B) reg_swap<T, A, B>(a)
         HGM_REG_BITS(HGM_CALL)
@@ -404,7 +412,7 @@ __global__ void __launch_bounds__(reg_kernel_max_threads(A)) k_pass_reg(typename
             }
             if ((base & o.gmask) != o.gval) continue;   // tile-uniform
             if (o.kind == R_PHASE || o.kind == R_PHASE_REAL) reg_phase<T, A>(a, o, t);
-            else reg_gate<T, A>(a, o, t);
+            else reg_gate<T, A, false>(a, o, t);
         }
 #pragma unroll
         for (int k = 0; k < A; ++k) state[base | st | pm->sreg[k]] = a[k];
@@ -459,7 +467,7 @@ __global__ void __launch_bounds__(reg_kernel_max_threads(A)) k_batch_reg(const B
         }
         if ((base & o.gmask) != o.gval) continue;
         if (o.kind == R_PHASE || o.kind == R_PHASE_REAL) reg_phase<T, A>(a, o, t);
-        else reg_gate<T, A>(a, o, t);
+        else reg_gate<T, A, true>(a, o, t);
     }
 #pragma unroll
     for (int k = 0; k < A; ++k) state[base | st | pm->sreg[k]] = a[k];
