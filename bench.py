@@ -403,6 +403,8 @@ def main():
     ap.add_argument("--seq-sample", type=int, default=256,
                     help="requests of the same stream run by the sequential FP64 baseline")
     ap.add_argument("--autotune", action="store_true", help="serving: autotune tile shapes per qubit count")
+    ap.add_argument("--autotune-tile", action="store_true",
+                    help="autotune the workload's tile shape (hgm_autotune, untimed) before the timed steps")
     ap.add_argument("--serving-tile", default=None, help="serving: B,R,r,c for states of 14 qubits and more")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg of the default run")
     ap.add_argument("--general-out", type=int, default=None, metavar="QUBITS",
@@ -492,10 +494,20 @@ def main():
         tun["ops_block"] = args.ops_block
     if args.kernel is not None:
         tun["kernel"] = args.kernel
-    tuning = N.default_tuning(**tun)
     n = w["qubits"]
     dtype = np.complex128 if w["dtype"] == "c128" else np.complex64
     d, ops, iterations, basis, name = build_circuit(w)
+    autotuned = None
+    if args.autotune_tile:   # time the kernel family's tile shapes on this circuit and keep the fastest
+        best, best_ms = N.Tuning(), C.c_double()
+        t_at = time.perf_counter()
+        N.check(N.lib.hgm_autotune(n, N.HGM_C128 if dtype == np.complex128 else N.HGM_C64, ops, len(ops), iterations,
+                                   tun.get("kernel", 0), tun.get("planner", 0), C.byref(best), C.byref(best_ms)))
+        tun.update(block_size=best.block_size, repeat_count=best.repeat_count, gpu_region_bits=best.gpu_region_bits,
+                   gpu_coalesced_bits=best.gpu_coalesced_bits)
+        autotuned = {"seconds": time.perf_counter() - t_at, "best_kernel_ms": best_ms.value}
+        print(f"bench: autotuned tile {tun} in {autotuned['seconds']:.1f} s", file=sys.stderr, flush=True)
+    tuning = N.default_tuning(**tun)
     plan = hq.Plan(n, ops, tuning, iterations, dtype)
     if args.graph:
         plan.set_graph(True)
@@ -543,7 +555,8 @@ def main():
         "dtype": "f64" if dtype == np.complex128 else "f32",
         "data": "synthetic (seeded circuit, state |0..0> resident in HBM)",
         "config": {"workload": name, "qubits": n, "gates_per_step": gates, "passes_per_step": len(plan),
-                   "tile": {k: v for k, v in tun.items()}, "planner": planner_name(tun.get("planner", 0)), "plan_compile_s": plan.compile_seconds, "state_norm_after": norm},
+                   "tile": {k: v for k, v in tun.items()}, "planner": planner_name(tun.get("planner", 0)), "plan_compile_s": plan.compile_seconds, "state_norm_after": norm,
+                   "tile_autotuned": autotuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": KERNEL_NAMES.get(plan.kernel, "?"),

@@ -239,13 +239,15 @@ int hgm_pivot_rows(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, in
 int hgm_validate_tuning(int qubits, int dtype, const hgm_tuning *t);
 
 /* ------------------------------------------------------------------ autotuner
- * Times every valid tile shape (2*repeat_count in {8, 16, 32} amplitudes per
- * thread, gpu_region_bits 8..13, gpu_coalesced_bits 2..4) of the given kernel
- * family (AUTO, REG or LDS) on a scratch device state running the given
- * circuit, and returns the fastest tuning and its kernel time. No counterpart
- * in the reference, whose tuning is given by hand on the command line
- * (include/cli/cli_common.h:21-25). */
-int hgm_autotune(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, int kernel,
+ * Times every valid tile shape of the given kernel family on a scratch device
+ * state running the given circuit planned with `planner` (HGM_PLAN_* flags),
+ * and returns the fastest tuning and its kernel time. Shapes: 2*repeat_count
+ * in {8, 16, 32} amplitudes per thread; for AUTO, REG and LDS gpu_region_bits
+ * 8..13 and gpu_coalesced_bits 2..4; for JIT, whose every candidate is
+ * compiled, gpu_region_bits 10..13 and the coalesced bits that make 128-byte
+ * runs (3 for FP64, 4 for FP32). No counterpart in the reference, whose tuning
+ * is given by hand on the command line (include/cli/cli_common.h:21-25). */
+int hgm_autotune(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, int kernel, int planner,
                  hgm_tuning *best, double *best_ms);
 
 /* ------------------------------------------------------------------ serving

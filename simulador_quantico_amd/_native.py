@@ -157,7 +157,7 @@ SIGNATURES = [
     ("hgm_revert_bits", C.c_int, [C.c_int, C.c_int]),
     ("hgm_generate", C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64), C.c_int, C.c_char_p, C.c_char_p, C.c_size_t,
                                C.POINTER(C.c_size_t)]),
-    ("hgm_autotune", C.c_int, [C.c_int, C.c_int, C.POINTER(Op), C.c_int64, C.c_int, C.c_int, C.POINTER(Tuning),
+    ("hgm_autotune", C.c_int, [C.c_int, C.c_int, C.POINTER(Op), C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(Tuning),
                                C.POINTER(C.c_double)]),
     ("hgm_server_default_config", None, [C.POINTER(ServerConfig)]),
     ("hgm_server_create", C.c_int, [C.POINTER(ServerConfig), C.POINTER(_P)]),

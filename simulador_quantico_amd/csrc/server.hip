@@ -591,7 +591,8 @@ int hgm_server_create(const hgm_server_config *cfg, hgm_server **out) {
             const std::vector<hgm_op> ops = timing_circuit(n, 4);
             hgm_tuning best;
             double ms;
-            if ((rc = hgm_autotune(n, cfg->dtype, ops.data(), (int64_t)ops.size(), 1, HGM_KERNEL_REG, &best, &ms)))
+            if ((rc = hgm_autotune(n, cfg->dtype, ops.data(), (int64_t)ops.size(), 1, HGM_KERNEL_REG,
+                                   kServingPlanner, &best, &ms)))
                 return rc;
             best.ops_block = 1 << 30;
             best.planner = kServingPlanner;
@@ -719,23 +720,22 @@ void hgm_server_destroy(hgm_server *s) {
     delete s;
 }
 
-int hgm_autotune(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, int kernel,
+int hgm_autotune(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, int kernel, int planner,
                  hgm_tuning *best, double *best_ms) {
     if (!best || (n_ops > 0 && !ops) || iterations < 1) { set_error("bad argument"); return HGM_E_ARG; }
-    if (kernel != HGM_KERNEL_AUTO && kernel != HGM_KERNEL_REG && kernel != HGM_KERNEL_LDS) {
-        set_error("the autotuner times the AUTO, REG and LDS kernel families");
-        return HGM_E_ARG;
-    }
+    if (kernel < HGM_KERNEL_AUTO || kernel > HGM_KERNEL_JIT) { set_error("unknown kernel family"); return HGM_E_ARG; }
+    const bool jit = kernel == HGM_KERNEL_JIT;
+    const int r_lo = jit ? 10 : 8, c_run = dtype == HGM_C64 ? 4 : 3;
     hgm_state *st = nullptr;
     int rc = hgm_state_create(qubits, dtype, &st);
     if (rc) return rc;
     double best_t = 1e300;
     bool found = false;
     for (int A : {8, 16, 32}) {
-        for (int r = 8; r <= 13; ++r) {
+        for (int r = r_lo; r <= 13; ++r) {
             const int64_t B = (int64_t(1) << r) / A;
             if (B < 64 || B > 1024 || (kernel != HGM_KERNEL_LDS && B > reg_kernel_max_threads(A))) continue;
-            for (int c = 2; c <= 4; ++c) {
+            for (int c = jit ? c_run : 2; c <= (jit ? c_run : 4); ++c) {
                 hgm_tuning t;
                 hgm_default_tuning(&t);
                 t.block_size = (int)B;
@@ -743,6 +743,7 @@ int hgm_autotune(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int it
                 t.gpu_region_bits = r;
                 t.gpu_coalesced_bits = c;
                 t.kernel = kernel;
+                t.planner = planner;
                 t.ops_block = 1 << 30;
                 if (hgm_validate_tuning(qubits, dtype, &t) != HGM_OK) continue;
                 hgm_plan *p = nullptr;

@@ -318,7 +318,7 @@ def test_gpu_entry_points_fail_cleanly_without_a_device():
     assert rc != 0 and b"device" in N.lib.hgm_last_error()
     ops = hq.make_ops([0], [0], [0], [np.eye(2)])
     best = N.Tuning()
-    assert N.lib.hgm_autotune(8, N.HGM_C128, ops, 1, 1, N.HGM_KERNEL_REG, C.byref(best), None) != 0
+    assert N.lib.hgm_autotune(8, N.HGM_C128, ops, 1, 1, N.HGM_KERNEL_REG, N.HGM_PLAN_GREEDY, C.byref(best), None) != 0
     bad = N.ServerConfig()
     N.lib.hgm_server_default_config(C.byref(bad))
     bad.max_active = 0

@@ -6,6 +6,7 @@ import sys
 import numpy as np
 import pytest
 
+import simulador_quantico_amd as hq
 from simulador_quantico_amd import _native as N
 from simulador_quantico_amd import serving as S
 from tests.test_serving_mix import oracle_run_job
@@ -107,11 +108,36 @@ def test_autotune_picks_a_valid_shape():
     best = N.Tuning()
     ms = C.c_double()
     N.check(N.lib.hgm_autotune(n, N.HGM_C64, S._op_ptr(job.ops), len(job.ops), 1, N.HGM_KERNEL_REG,
+                               N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT,
                                C.byref(best), C.byref(ms)))
     say("autotune", n, best.block_size, best.repeat_count, best.gpu_region_bits, best.gpu_coalesced_bits, ms.value)
     assert ms.value > 0
     assert 2 * best.block_size * best.repeat_count == 1 << best.gpu_region_bits
     assert N.lib.hgm_validate_tuning(n, N.HGM_C64, C.byref(best)) == 0
+
+
+def test_autotune_plan_compiled_family():
+    """The autotuner compiles and times plan-compiled candidates with the
+    given planner flags; the tile it returns runs the circuit correctly."""
+    import ctypes as C
+    from tests.test_oracle import oracle_run_gates
+    n = 14
+    d = hq.DGM()
+    d.setFunction(hq.RandomLayers(n, 4, 7, d))
+    ops = d.ops()
+    flags = N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT
+    best = N.Tuning()
+    ms = C.c_double()
+    N.check(N.lib.hgm_autotune(n, N.HGM_C128, ops, len(ops), 1, N.HGM_KERNEL_JIT, flags, C.byref(best), C.byref(ms)))
+    say("autotune jit", best.block_size, best.repeat_count, best.gpu_region_bits, best.gpu_coalesced_bits, ms.value)
+    assert best.kernel == N.HGM_KERNEL_JIT and best.planner == flags and best.gpu_coalesced_bits == 3
+    assert 10 <= best.gpu_region_bits <= 13
+    s = hq.State(n)
+    s.set_basis(0)
+    hq.Plan(n, ops, best).run(s)
+    t, cm, cv, mats = hq.ops_to_arrays(ops)
+    want = oracle_run_gates(n, t, cm, cv, mats, basis=0)
+    assert np.abs(s.download() - want).max() < 1e-12
 
 
 def test_tuning_override_per_qubit_count():
