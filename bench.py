@@ -218,7 +218,7 @@ def fidelity_report(names=("hadamard20", "grover22", "draper29", "random30", "ra
     return out
 
 
-def drop_in_rate(name: str = "random30") -> dict:
+def drop_in_rate(name: str = "random30", tuning: dict | None = None) -> dict:
     """The workload through the drop-in executor (DGM with exec_type t_GPU and a
     pinned host state; every execute() uploads the state, plans, runs the
     passes and downloads, as the reference's t_GPU does): the PCIe-inclusive
@@ -227,14 +227,22 @@ def drop_in_rate(name: str = "random30") -> dict:
     w = WORKLOADS[name]
     d, ops, iterations, basis, title = build_circuit(w)
     d.exec_type = hq.T_GPU
-    for k, v in w["tuning"].items():
+    tun = dict(w["tuning"]) if tuning is None else tuning
+    for k, v in tun.items():
         setattr(d, k, v)
     d.allocateMemory()
-    d.setMemoryValue(basis)
+    grover = w["kind"] == "grover"
+
+    def init():
+        if grover:   # Grover starts from the uniform superposition (the H layer, set on the host)
+            d.state[:] = 2.0 ** (-w["qubits"] / 2)
+        else:
+            d.setMemoryValue(basis)
+    init()
     t0 = time.perf_counter()
     d.execute(iterations)                 # first call: device buffer, kernel compilation
     cold = time.perf_counter() - t0
-    d.setMemoryValue(basis)
+    init()
     t0 = time.perf_counter()
     d.execute(iterations)
     wall = time.perf_counter() - t0
@@ -243,7 +251,12 @@ def drop_in_rate(name: str = "random30") -> dict:
     out = {"workload": title, "dtype": "f64" if w["dtype"] == "c128" else "f32",
            "amp_updates_per_s": gates * float(1 << w["qubits"]) / wall, "wall_ms": wall * 1e3,
            "first_call_ms": cold * 1e3, "h2d_ms": st["h2d_ms"], "kernel_ms": st["kernel_ms"], "d2h_ms": st["d2h_ms"],
-           "plan_ms": st["plan_ms"]}
+           "plan_ms": st["plan_ms"], "tuning": tun}
+    if grover:   # readout on the host state the drop-in returns (untimed)
+        prob = np.abs(d.state) ** 2
+        idx = int(np.argmax(prob))
+        marked = int(np.random.default_rng(w["seed"]).integers(0, 1 << w["qubits"]))
+        out.update(argmax=idx, marked=marked, argmax_exact=idx == marked, p_max=float(prob[idx]))
     d.freeMemory()
     return out
 
@@ -451,7 +464,17 @@ def main():
         return
 
     if args.drop_in:
-        print(json.dumps({"drop_in": drop_in_rate(args.workload)}))
+        tun = None
+        if args.tuning or args.kernel is not None or args.planner is not None:
+            tun = dict(WORKLOADS[args.workload]["tuning"])
+            if args.tuning:
+                B, R, r, c = (int(x) for x in args.tuning.split(","))
+                tun.update(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c)
+            if args.kernel is not None:
+                tun["kernel"] = args.kernel
+            if args.planner is not None:
+                tun["planner"] = args.planner
+        print(json.dumps({"drop_in": drop_in_rate(args.workload, tun)}))
         return
 
     if args.fidelity:

@@ -144,15 +144,19 @@ int hgm_init(int device);
 /* Number of visible HIP devices (0 when none). */
 int hgm_device_count(int *count);
 /* Free the device buffers the library caches between calls (the hgm_execute
- * state buffer, op tables, register programs) and unload plan-compiled kernel
- * modules; they are rebuilt on demand. */
+ * state buffer, op tables, register programs), drop the plans that hgm_execute
+ * and hgm_state_apply keep for the last 4 distinct (gate list, tuning,
+ * iterations, qubits, dtype) they ran, and unload plan-compiled kernel
+ * modules; all are rebuilt on demand. */
 int hgm_release_cache(void);
 
 /* ------------------------------------------------------------------ one-shot
  * Applies ops[0..n_ops) `iterations` times to a host state of 2^qubits
  * amplitudes of type dtype, in place: upload, run every pass, download. This
  * is what the reference's t_GPU does per execute() (src/core/kernel.cu:131-291),
- * with the device buffer cached across calls. */
+ * with the device buffer cached across calls, and the plan too when the same
+ * gate list, tuning and iterations ran recently (stats->plan_ms then holds the
+ * lookup time). */
 int hgm_execute(void *host_state, int qubits, int dtype, const hgm_op *ops, int64_t n_ops,
                 const hgm_tuning *tuning, int iterations, hgm_stats *stats);
 

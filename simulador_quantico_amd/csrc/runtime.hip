@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <list>
 #include <atomic>
 #include <thread>
 #include <chrono>
@@ -214,7 +215,7 @@ int launch_jit_pass_locked(hgm_state *s, const Plan &p, JitModule *jm, size_t pi
     return HGM_OK;
 }
 
-void add_stats(hgm_stats *st, const Plan &p, float kernel_ms) {
+void add_stats(hgm_stats *st, const Plan &p, float kernel_ms, double plan_ms) {
     if (!st) return;
     int64_t launched = 0;
     for (const Pass &ps : p.passes) launched += ps.op_count > 0;
@@ -222,7 +223,7 @@ void add_stats(hgm_stats *st, const Plan &p, float kernel_ms) {
     st->passes += launched;
     st->amp_updates += p.gates * (int64_t(1) << p.qubits);
     st->bytes_alg += (double)launched * 2.0 * (double)(uint64_t(1) << p.qubits) * (double)amp_bytes(p.dtype);
-    st->plan_ms += p.plan_ms;
+    st->plan_ms += plan_ms;
     st->kernel_ms += kernel_ms;
 }
 
@@ -264,12 +265,12 @@ int run_plan_graph_locked(hgm_state *s, const hgm_plan *hp, hgm_stats *st) {
     HGM_HIP(hipGraphLaunch(hp->graph_exec, c.stream), "hipGraphLaunch");
     HGM_HIP(hipEventRecord(c.e1, c.stream), "hipEventRecord");
     HGM_HIP(hipEventSynchronize(c.e1), "pass execution");
-    add_stats(st, p, elapsed(c.e0, c.e1));
+    add_stats(st, p, elapsed(c.e0, c.e1), p.plan_ms);
     return HGM_OK;
 }
 
 // Launches every pass of the plan on the context stream. Assumes the lock.
-int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st) {
+int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st, double plan_ms) {
     Ctx &c = ctx();
     if (s->qubits != p.qubits || s->dtype != p.dtype) {
         t_err = "plan was built for another qubit count or dtype";
@@ -328,7 +329,7 @@ int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st) {
     }
     HGM_HIP(hipEventRecord(c.e1, c.stream), "hipEventRecord");
     HGM_HIP(hipEventSynchronize(c.e1), "pass execution");
-    add_stats(st, p, elapsed(c.e0, c.e1));
+    add_stats(st, p, elapsed(c.e0, c.e1), plan_ms);
     return HGM_OK;
 }
 
@@ -394,17 +395,74 @@ void free_pinned(void *p) {
     if (p) hipHostFree(p);
 }
 
+// Plans built by the one-call entry points (hgm_execute, hgm_state_apply) for
+// the last few (gate list, tuning, iterations, qubits, dtype): a caller that
+// runs the same circuit again, as the drop-in executor does on every
+// execute(), skips planning and code generation. Keys hold the whole gate
+// list and are compared field by field.
+struct CachedPlan {
+    int qubits, dtype, iterations;
+    hgm_tuning t;
+    std::vector<hgm_op> ops;
+    std::shared_ptr<const Plan> plan;
+};
+constexpr size_t kPlanCacheEntries = 4;
+std::mutex g_plan_cache_mu;
+std::list<CachedPlan> g_plan_cache;   // most recently used first
+
+bool same_tuning(const hgm_tuning &a, const hgm_tuning &b) {
+    return a.block_size == b.block_size && a.repeat_count == b.repeat_count &&
+           a.gpu_region_bits == b.gpu_region_bits && a.gpu_coalesced_bits == b.gpu_coalesced_bits &&
+           a.ops_block == b.ops_block && a.planner == b.planner && a.kernel == b.kernel && a.lookahead == b.lookahead;
+}
+
+bool same_ops(const std::vector<hgm_op> &a, const hgm_op *b, int64_t n) {
+    if ((int64_t)a.size() != n) return false;
+    for (int64_t i = 0; i < n; ++i)
+        if (a[i].target_bit != b[i].target_bit || a[i].control_mask != b[i].control_mask ||
+            a[i].control_value != b[i].control_value || std::memcmp(a[i].m, b[i].m, sizeof(a[i].m)) != 0)
+            return false;
+    return true;
+}
+
+int cached_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, const hgm_tuning &t,
+                std::shared_ptr<const Plan> *out, double *plan_ms) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (n_ops < 0 || (n_ops > 0 && !ops)) { t_err = "bad op array"; return HGM_E_ARG; }
+    {
+        std::lock_guard<std::mutex> g(g_plan_cache_mu);
+        for (auto it = g_plan_cache.begin(); it != g_plan_cache.end(); ++it)
+            if (it->qubits == qubits && it->dtype == dtype && it->iterations == iterations && same_tuning(it->t, t) &&
+                same_ops(it->ops, ops, n_ops)) {
+                g_plan_cache.splice(g_plan_cache.begin(), g_plan_cache, it);
+                *out = g_plan_cache.front().plan;
+                *plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                return HGM_OK;
+            }
+    }
+    auto p = std::make_shared<Plan>();
+    int rc = build_plan(qubits, dtype, ops, n_ops, iterations, t, p.get(), &t_err);
+    if (rc) return rc;
+    *plan_ms = p->plan_ms;
+    std::lock_guard<std::mutex> g(g_plan_cache_mu);
+    g_plan_cache.push_front(CachedPlan{qubits, dtype, iterations, t, std::vector<hgm_op>(ops, ops + n_ops), p});
+    if (g_plan_cache.size() > kPlanCacheEntries) g_plan_cache.pop_back();
+    *out = std::move(p);
+    return HGM_OK;
+}
+
 int plan_and_run(hgm_state *s, const hgm_op *ops, int64_t n_ops, const hgm_tuning *tuning, int iterations,
                  hgm_stats *st) {
     hgm_tuning t;
     if (tuning) t = *tuning; else hgm_default_tuning(&t);
-    Plan p;
-    int rc = build_plan(s->qubits, s->dtype, ops, n_ops, iterations, t, &p, &t_err);
+    std::shared_ptr<const Plan> p;
+    double plan_ms = 0;
+    int rc = cached_plan(s->qubits, s->dtype, ops, n_ops, iterations, t, &p, &plan_ms);
     if (rc) return rc;
     std::lock_guard<std::mutex> g(ctx().mu);
     rc = init_locked(-1);
     if (rc) return rc;
-    return run_plan_locked(s, p, st);
+    return run_plan_locked(s, *p, st, plan_ms);
 }
 
 }  // namespace hgm
@@ -438,6 +496,10 @@ int hgm_device_count(int *count) {
 }
 
 int hgm_release_cache(void) {
+    {
+        std::lock_guard<std::mutex> g(g_plan_cache_mu);
+        g_plan_cache.clear();
+    }
     Ctx &c = ctx();
     std::lock_guard<std::mutex> g(c.mu);
     if (!c.ready) return HGM_OK;
@@ -758,7 +820,7 @@ int hgm_state_run_plan(hgm_state *s, const hgm_plan *p, hgm_stats *stats) {
     auto t0 = std::chrono::steady_clock::now();
     std::lock_guard<std::mutex> g(ctx().mu);
     int rc = init_locked(-1);
-    if (!rc) rc = p->graph_on ? run_plan_graph_locked(s, p, stats) : run_plan_locked(s, p->plan, stats);
+    if (!rc) rc = p->graph_on ? run_plan_graph_locked(s, p, stats) : run_plan_locked(s, p->plan, stats, p->plan.plan_ms);
     if (stats) stats->total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
 }
@@ -781,9 +843,11 @@ int hgm_execute(void *host_state, int qubits, int dtype, const hgm_op *ops, int6
     auto t0 = std::chrono::steady_clock::now();
     hgm_tuning t;
     if (tuning) t = *tuning; else hgm_default_tuning(&t);
-    Plan p;
-    int rc = build_plan(qubits, dtype, ops, n_ops, iterations, t, &p, &t_err);
+    std::shared_ptr<const Plan> plan;
+    double plan_ms = 0;
+    int rc = cached_plan(qubits, dtype, ops, n_ops, iterations, t, &plan, &plan_ms);
     if (rc) return rc;
+    const Plan &p = *plan;
     Ctx &c = ctx();
     std::lock_guard<std::mutex> g(c.mu);
     rc = init_locked(-1);
@@ -807,7 +871,7 @@ int hgm_execute(void *host_state, int qubits, int dtype, const hgm_op *ops, int6
     HGM_HIP(hipEventRecord(c.e2, c.stream), "hipEventRecord");
     HGM_HIP(hipMemcpyAsync(s.d, host_state, bytes, hipMemcpyHostToDevice, c.stream), "H2D state");
     HGM_HIP(hipEventRecord(c.e3, c.stream), "hipEventRecord");
-    rc = run_plan_locked(&s, p, stats);
+    rc = run_plan_locked(&s, p, stats, plan_ms);
     if (rc) return rc;
     HGM_HIP(hipEventRecord(c.e4, c.stream), "hipEventRecord");
     HGM_HIP(hipMemcpyAsync(host_state, s.d, bytes, hipMemcpyDeviceToHost, c.stream), "D2H state");

@@ -380,3 +380,35 @@ def test_jit_graph_replay_matches_direct_launches():
     with pytest.raises(N.HGMError):
         hq.Plan(n, ops, N.default_tuning(block_size=64, repeat_count=4, gpu_region_bits=9,
                                          kernel=N.HGM_KERNEL_REG)).set_graph(True)
+
+
+def test_one_call_entry_reuses_plans_only_for_the_same_circuit():
+    """hgm_execute keeps recent plans: the same gate list runs again without
+    planning, and a list that differs in one matrix entry gets its own plan."""
+    import ctypes as C
+    n = 12
+    rng = np.random.default_rng(31)
+    t, cm, cv, mats = _random_gate_list(rng, n, 60)
+    tun = N.default_tuning(block_size=64, repeat_count=4, gpu_region_bits=9, gpu_coalesced_bits=3,
+                           kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT)
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+
+    def run(mats_):
+        ops = hq.make_ops(t, cm, cv, mats_)
+        psi = psi0.astype(np.complex128).copy()
+        st = N.Stats()
+        N.check(N.lib.hgm_execute(psi.ctypes.data, n, N.HGM_C128, ops, len(ops), C.byref(tun), 2, C.byref(st)))
+        return psi, st.plan_ms
+
+    first, ms1 = run(mats)
+    again, ms2 = run(mats)
+    assert np.array_equal(first, again)
+    assert ms2 < 0.2 * ms1 + 1.0, (ms1, ms2)
+    mats2 = [m.copy() for m in mats]
+    k = next(i for i, m in enumerate(mats2) if m[1] != 0)
+    rot = np.array([[np.cos(0.3), -np.sin(0.3)], [np.sin(0.3), np.cos(0.3)]])
+    mats2[k] = (np.asarray(mats2[k]).reshape(2, 2) @ rot).reshape(4)
+    other, _ = run(mats2)
+    want = oracle_run_gates(n, np.tile(t, 2), np.tile(cm, 2), np.tile(cv, 2), list(mats2) * 2, state=psi0)
+    assert np.abs(other - want).max() < 1e-12
