@@ -320,7 +320,9 @@ def test_jit_kernel_parity(dtype, tol):
                                 (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FOLD_PHASES),
                                 (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_FOLD_PHASES),
                                 (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_PIVOT),
-                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT)]:
+                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT),
+                                (64, 8, 10, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT |
+                                 N.HGM_PLAN_BALANCE)]:
         tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c,
                                kernel=N.HGM_KERNEL_JIT, ops_block=60, planner=planner)
         plan = hq.Plan(n, ops, tun, 1, dtype)
@@ -332,13 +334,17 @@ def test_jit_kernel_parity(dtype, tol):
         assert np.abs(s.download() - want).max() < tol
 
 
-@pytest.mark.parametrize("flag", [N.HGM_PLAN_FOLD_PHASES, N.HGM_PLAN_PIVOT])
-def test_jit_random_layers_30q_round_trip(flag):
+@pytest.mark.parametrize("flag,tile", [(N.HGM_PLAN_FOLD_PHASES, (256, 4, 11, 3)),
+                                       (N.HGM_PLAN_PIVOT | N.HGM_PLAN_BALANCE, (256, 8, 12, 3))])
+def test_jit_random_layers_30q_round_trip(flag, tile):
+    """The benchmark's 30-qubit circuit and configuration: U then U^dagger
+    returns |0> and the norm stays 1 (size-independent checks at full size)."""
     n = 30
     d = hq.DGM()
     d.setFunction(hq.RandomLayers(n, 20, 1, d))
     ops = d.ops()
-    tun = N.default_tuning(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
+    B, R, r, c = tile
+    tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c,
                            kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | flag)
     s = hq.State(n)
     s.set_basis(0)
