@@ -46,14 +46,16 @@ def test_truth_tables(dtype):
         assert np.abs(d.state - want).max() < 1e-6, (circ, start)
 
 
-def test_golden_circuits(golden):
-    """FP32 path vs the reference's FP32 states; FP64 path vs the FP64 oracle."""
+@pytest.mark.parametrize("planner", [N.HGM_PLAN_GREEDY, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT])
+def test_golden_circuits(golden, planner):
+    """FP32 path vs the reference's FP32 states; FP64 path vs the FP64 oracle;
+    with and without the fusion and row-scaling rewrites."""
     for c in golden["circuits"]:
-        say("golden", c["name"])
+        say("golden", c["name"], planner)
         n = c["qubits"]
         ref32 = golden["states"][f"{c['name']}__tcpu"]
         for dtype in (np.complex64, np.complex128):
-            d = dgm_for(n, dtype)
+            d = dgm_for(n, dtype, planner=planner)
             for name, m in c["gates"].items():
                 d.gates.addGate(name, [complex(a, b) for a, b in m])
             d.setMemoryValue(c["basis"])
