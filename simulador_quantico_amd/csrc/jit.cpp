@@ -255,6 +255,28 @@ std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const st
     return o;
 }
 
+// Everything kernel_body reads for one pass, as bytes: passes with equal keys
+// (the repeated iterations of a circuit) have equal bodies, so only the first
+// is generated. Map indices are made relative to the pass's first map.
+std::string pass_key(const Plan &plan, const Pass &ps) {
+    const PassMem &pm = plan.mems[ps.mem];
+    std::string k;
+    auto put = [&](const void *p, size_t n) { k.append(static_cast<const char *>(p), n); };
+    put(&ps.local_mask, sizeof(ps.local_mask));
+    put(&ps.phases, sizeof(ps.phases));
+    PassMem m = pm;
+    m.first_map = 0;
+    m.last_map = pm.last_map - pm.first_map;
+    put(&m, sizeof(m));
+    for (int32_t mi = pm.first_map; mi <= pm.last_map; ++mi) put(&plan.maps[mi], sizeof(PhaseMap));
+    for (int64_t i = 0; i < ps.rop_count; ++i) {
+        RegOp o = plan.rops[ps.rop_begin + i];
+        if (o.kind == R_REMAP) o.rbit -= pm.first_map;
+        put(&o, sizeof(o));
+    }
+    return k;
+}
+
 struct Compiled {
     std::string source;          // compared on lookup: the hash only narrows the search
     std::vector<char> binary;
@@ -283,15 +305,24 @@ void jit_generate(const Plan &plan, JitCode *out) {
                   "struct __attribute__((aligned(8))) hgm_f2 { float x, y; };\n";
     out->pass_kernel.clear();
     out->names.clear();
-    std::map<std::string, int> seen;   // body (with a placeholder name) -> kernel index
+    std::map<std::string, int> seen;                 // body (with a placeholder name) -> kernel index
+    std::unordered_map<std::string, int> seen_key;   // pass_key -> kernel index
     for (const Pass &ps : plan.passes) {
         if (ps.op_count == 0) { out->pass_kernel.push_back(-1); continue; }
+        std::string key = pass_key(plan, ps);
+        auto kt = seen_key.find(key);
+        if (kt != seen_key.end()) { out->pass_kernel.push_back(kt->second); continue; }
         std::string body = kernel_body(g, plan, ps, "@NAME@");
         auto it = seen.find(body);
-        if (it != seen.end()) { out->pass_kernel.push_back(it->second); continue; }
+        if (it != seen.end()) {
+            seen_key.emplace(std::move(key), it->second);
+            out->pass_kernel.push_back(it->second);
+            continue;
+        }
         const int idx = (int)out->names.size();
         const std::string name = "hgm_pass_" + std::to_string(idx);
         seen[body] = idx;
+        seen_key.emplace(std::move(key), idx);
         out->names.push_back(name);
         out->pass_kernel.push_back(idx);
         body.replace(body.find("@NAME@"), 6, name);
