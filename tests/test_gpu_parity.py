@@ -420,3 +420,29 @@ def test_one_call_entry_reuses_plans_only_for_the_same_circuit():
     other, _ = run(mats2)
     want = oracle_run_gates(n, np.tile(t, 2), np.tile(cm, 2), np.tile(cv, 2), list(mats2) * 2, state=psi0)
     assert np.abs(other - want).max() < 1e-12
+
+
+def test_release_cache_then_reuse():
+    """hgm_release_cache frees the cached device buffers, drops cached plans
+    and unloads compiled modules; the next calls rebuild them and give the
+    same results, for the one-call path and for a plan built before."""
+    import ctypes as C
+    n = 13
+    rng = np.random.default_rng(5)
+    t, cm, cv, mats = _random_gate_list(rng, n, 40)
+    ops = hq.make_ops(t, cm, cv, mats)
+    tun = N.default_tuning(block_size=64, repeat_count=4, gpu_region_bits=9, gpu_coalesced_bits=3,
+                           kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT)
+    want = oracle_run_gates(n, t, cm, cv, mats, basis=3)
+    plan = hq.Plan(n, ops, tun)
+    for _ in range(2):
+        psi = np.zeros(1 << n, np.complex128)
+        psi[3] = 1
+        N.check(N.lib.hgm_execute(psi.ctypes.data, n, N.HGM_C128, ops, len(ops), C.byref(tun), 1, None))
+        assert np.abs(psi - want).max() < 1e-12
+        s = hq.State(n)
+        s.set_basis(3)
+        plan.run(s)
+        assert np.abs(s.download() - want).max() < 1e-12
+        del s
+        N.check(N.lib.hgm_release_cache())
