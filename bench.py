@@ -261,6 +261,38 @@ def drop_in_rate(name: str = "random30", tuning: dict | None = None) -> dict:
     return out
 
 
+def other_configs() -> dict:
+    """Short checks of the other benchmark configurations, run after the timed
+    steps: Hadamard 20q against its closed form (every amplitude 2^-10),
+    Grover 22q through the drop-in executor (argmax = the marked value), and
+    the Draper adder 29q (the sum register read bit-exact)."""
+    import simulador_quantico_amd as hq
+    from simulador_quantico_amd import _native as N
+    out = {}
+    w = WORKLOADS["hadamard20"]
+    d, ops, iterations, basis, title = build_circuit(w)
+    s = hq.State(w["qubits"])
+    s.set_basis(basis)
+    st = hq.Plan(w["qubits"], ops, N.default_tuning(**w["tuning"]), iterations).run(s)
+    err = float(np.abs(s.download() - 2.0 ** (-w["qubits"] / 2)).max())
+    out["hadamard20"] = {"circuit": title, "kernel_ms": st["kernel_ms"], "max_abs_err_vs_closed_form": err}
+    del s
+    out["grover22_drop_in"] = drop_in_rate("grover22")
+    w = WORKLOADS["draper29"]
+    d, ops, iterations, basis, title = build_circuit(w)
+    width = (w["qubits"] - 1) // 2
+    a, b = (int(v) for v in np.random.default_rng(w["seed"]).integers(0, 1 << width, 2))
+    s = hq.State(w["qubits"])
+    s.set_basis(basis)
+    st = hq.Plan(w["qubits"], ops, N.default_tuning(**w["tuning"]), iterations).run(s)
+    idx, p = s.argmax()
+    out["draper29"] = {"circuit": title, "kernel_ms": st["kernel_ms"], "a": a, "b": b,
+                       "sum_read": idx & ((1 << (width + 1)) - 1), "sum_exact": idx == ((a << (width + 1)) | (a + b)),
+                       "p_max": p}
+    del s
+    return out
+
+
 def general_out(qubits: int, tuning: dict) -> dict:
     """The reference's general.out timing (src/cli/general.cpp:42-52 around
     HadamardNQubits, src/algorithms/lib_general.cpp:15-47): executor creation,
@@ -427,6 +459,8 @@ def main():
     ap.add_argument("--drop-in", action="store_true",
                     help="run only the workload through the drop-in executor (PCIe-inclusive rate)")
     ap.add_argument("--no-fidelity", action="store_true", help="skip the FP32-vs-FP64 leg of the default run")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the checks of the Hadamard, Grover and Draper configurations in the default run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain device copy of the workload's state (the known-size reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters) instead of the circuit")
@@ -616,6 +650,11 @@ def main():
                                            autotune=args.autotune)
         except Exception as e:
             out["serving"] = {"error": str(e)}
+    if default_run and not args.no_configs:
+        try:
+            out["other_configs"] = other_configs()
+        except Exception as e:
+            out["other_configs"] = {"error": str(e)}
     if default_run and not args.no_fidelity:
         try:
             out["fidelity_fp32_vs_fp64"] = fidelity_report()
