@@ -80,7 +80,7 @@ WORKLOADS = {
                      tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=4,
                                  kernel=3, planner=96)),
     "draper29": dict(qubits=29, dtype="c128", kind="draper", seed=29,
-                     tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
+                     tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=3,
                                  kernel=3, planner=96)),
 }
 
