@@ -446,3 +446,42 @@ def test_release_cache_then_reuse():
         assert np.abs(s.download() - want).max() < 1e-12
         del s
         N.check(N.lib.hgm_release_cache())
+
+
+def test_one_call_entry_from_several_threads():
+    """Four threads run two different circuits through hgm_execute at once
+    (the plan cache and the device context are shared); every result matches
+    the oracle."""
+    import ctypes as C
+    import threading
+    n = 11
+    rng = np.random.default_rng(8)
+    circuits = []
+    for _ in range(2):
+        t, cm, cv, mats = _random_gate_list(rng, n, 50)
+        want = oracle_run_gates(n, t, cm, cv, mats, basis=1)
+        circuits.append((hq.make_ops(t, cm, cv, mats), want))
+    tun = N.default_tuning(block_size=64, repeat_count=4, gpu_region_bits=9, gpu_coalesced_bits=3,
+                           planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT)
+    errors = []
+
+    def worker(w):
+        try:
+            for i in range(6):
+                ops, want = circuits[(w + i) % 2]
+                psi = np.zeros(1 << n, np.complex128)
+                psi[1] = 1
+                N.check(N.lib.hgm_execute(psi.ctypes.data, n, N.HGM_C128, ops, len(ops), C.byref(tun), 1, None))
+                err = float(np.abs(psi - want).max())
+                if err >= 1e-12:
+                    errors.append((w, i, err))
+        except Exception as e:   # reported by the main thread
+            errors.append((w, repr(e)))
+
+    threads = [threading.Thread(target=worker, args=(w,)) for w in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=120)
+    assert not any(th.is_alive() for th in threads)
+    assert not errors, errors
