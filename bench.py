@@ -71,10 +71,10 @@ WORKLOADS = {
                        tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4,
                                    planner=96)),
     "hadamard30x3": dict(qubits=30, dtype="c128", kind="hadamard", layers=3,
-                         tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
+                         tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=3,
                                      kernel=3, planner=0)),
     "hadamard30x3f": dict(qubits=30, dtype="c64", kind="hadamard", layers=3,
-                          tuning=dict(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=4,
+                          tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=4,
                                       kernel=3, planner=0)),
     "grover22": dict(qubits=22, dtype="c128", kind="grover", seed=22,
                      tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=4,
@@ -87,8 +87,8 @@ WORKLOADS = {
 
 def planner_name(flags: int) -> str:
     base = "window" if (flags & 15) == 1 else "greedy"
-    extra = [n for bit, n in ((32, "gate fusion"), (64, "row scaling"), (16, "phase folding"), (128, "pass balancing"))
-             if flags & bit and not (bit == 16 and flags & 64)]
+    extra = [n for bit, n in ((32, "gate fusion"), (64, "row scaling"), (16, "phase folding"), (128, "pass balancing"),
+                              (256, "two-pass lookahead")) if flags & bit and not (bit == 16 and flags & 64)]
     return " + ".join([base] + extra)
 
 

@@ -78,7 +78,12 @@ enum { HGM_PLAN_GREEDY = 0, /* dependency-aware lookahead, any set of tile-local
        /* flag (greedy planner): after packing, move gates between neighbouring passes from the one
         * with more arithmetic to the one with less, where the gate commutes with the gates it passes
         * and its target is tile-local there; the pass count is unchanged. */
-       HGM_PLAN_BALANCE = 128 };
+       HGM_PLAN_BALANCE = 128,
+       /* flag (greedy planner): choose each pass's tile-local set by the gates it admits plus the
+        * most the next pass could then admit, instead of by the gates it admits alone. Fewer passes
+        * on layered circuits (30-qubit random layers, 2^12 tiles: 11 instead of 12), at about
+        * 20 times the planning work. */
+       HGM_PLAN_LOOKAHEAD = 256 };
 
 /* kernel families */
 enum { HGM_KERNEL_AUTO = 0,  /* the register kernel when the tile shape allows it, else LDS */

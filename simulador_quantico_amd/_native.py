@@ -21,6 +21,7 @@ HGM_PLAN_FOLD_PHASES = 16
 HGM_PLAN_FUSE = 32
 HGM_PLAN_PIVOT = 64
 HGM_PLAN_BALANCE = 128
+HGM_PLAN_LOOKAHEAD = 256
 HGM_KERNEL_AUTO = 0
 HGM_KERNEL_LDS = 1
 HGM_KERNEL_REG = 2

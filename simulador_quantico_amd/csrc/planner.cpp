@@ -205,7 +205,8 @@ int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::
     if (B > 1024) { *err = "Erro de tuning: block_size must be <= 1024 on gfx950"; return HGM_E_TUNING; }
     if (!is_pow2(B) || !is_pow2(R)) { *err = "Erro de tuning: block_size and repeat_count must be powers of two"; return HGM_E_TUNING; }
     if (t.ops_block < 1) { *err = "Erro de tuning: ops_block must be >= 1"; return HGM_E_TUNING; }
-    const int base_planner = t.planner & ~(HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE | HGM_PLAN_PIVOT | HGM_PLAN_BALANCE);
+    const int base_planner =
+        t.planner & ~(HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE | HGM_PLAN_PIVOT | HGM_PLAN_BALANCE | HGM_PLAN_LOOKAHEAD);
     if (base_planner != HGM_PLAN_GREEDY && base_planner != HGM_PLAN_WINDOW) {
         *err = "unknown planner";
         return HGM_E_TUNING;
@@ -550,13 +551,13 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
         std::vector<char> done((size_t)N, 0);
         std::vector<int64_t> cand;
         int64_t head = 0;
-        auto scan = [&](uint64_t fixed, bool grow, uint64_t *local_out, std::vector<int64_t> &out) {
+        auto scan_from = [&](int64_t from, uint64_t fixed, bool grow, uint64_t *local_out, std::vector<int64_t> &out) {
             out.clear();
             uint64_t local = grow ? low : fixed;
             int nlocal = __builtin_popcountll(local);
             uint64_t bw = 0, br = 0;   // writes / reads of the skipped gates
             int64_t scanned = 0;
-            for (int64_t k = head; k < N && (int64_t)out.size() < cap && scanned < window; ++k) {
+            for (int64_t k = from; k < N && (int64_t)out.size() < cap && scanned < window; ++k) {
                 if (done[k]) continue;
                 ++scanned;
                 const GateInfo &g = info[k % n_ops];
@@ -576,17 +577,45 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
             *local_out = local;
         };
         const int e = r - c;
+        // the candidate local sets of a pass: grown on demand (index 0), or one window per index
+        const int n_cands = 1 + std::max(0, qubits - e - c + 1);
+        auto cand_scan = [&](int64_t from, int i, uint64_t *local_out, std::vector<int64_t> &out) {
+            if (i == 0) scan_from(from, 0, true, local_out, out);
+            else scan_from(from, low | (((uint64_t(1) << e) - 1) << (c + i - 1)), false, local_out, out);
+        };
+        const bool lookahead = (t.planner & HGM_PLAN_LOOKAHEAD) != 0;
+        std::vector<int64_t> next;
+        // the most gates one pass could take once `taken` is done (marked in place, then unmarked)
+        auto best_next = [&](const std::vector<int64_t> &taken) -> size_t {
+            for (int64_t k : taken) done[k] = 1;
+            int64_t h = head;
+            while (h < N && done[h]) ++h;
+            size_t best = 0;
+            uint64_t l;
+            for (int i = 0; h < N && i < n_cands; ++i) {
+                cand_scan(h, i, &l, next);
+                best = std::max(best, next.size());
+            }
+            for (int64_t k : taken) done[k] = 0;
+            return best;
+        };
         std::vector<std::vector<int64_t>> lists;
         std::vector<Pass> shells;
         std::vector<uint64_t> locals;
         while (head < N) {
-            uint64_t local;
-            scan(0, true, &local, chosen);
-            for (int s0 = c; s0 + e <= qubits; ++s0) {
-                const uint64_t L = low | (((uint64_t(1) << e) - 1) << s0);
-                uint64_t l2;
-                scan(L, false, &l2, cand);
-                if (cand.size() > chosen.size()) { chosen.swap(cand); local = L; }
+            uint64_t local = 0;
+            size_t best_value = 0;
+            chosen.clear();
+            for (int i = 0; i < n_cands; ++i) {
+                uint64_t L;
+                cand_scan(head, i, &L, cand);
+                // plain greedy: the most gates now; lookahead: the most over this pass and the next
+                const size_t value = lookahead ? cand.size() + best_next(cand) : cand.size();
+                if (i == 0 || value > best_value || (value == best_value && cand.size() > chosen.size())) {
+                    best_value = value;
+                    chosen.swap(cand);
+                    local = L;
+                }
             }
             for (int64_t k : chosen) done[k] = 1;
             while (head < N && done[head]) ++head;

@@ -199,3 +199,22 @@ def test_balance_keeps_passes_and_gates():
         for i in idx:
             if not (mats[i][1] == 0 and mats[i][2] == 0):
                 assert (mask >> int(t[i])) & 1
+
+
+def test_lookahead_saves_passes_on_random_layers():
+    """Choosing each pass's local set by what it and the next pass admit
+    (HGM_PLAN_LOOKAHEAD) never needs more passes than plain greedy here, and
+    needs fewer on the 30-qubit depth-20 random-layer circuit."""
+    n = 30
+    d = hq.DGM()
+    d.setFunction(hq.RandomLayers(n, 20, 1, d))
+    ops = d.ops()
+    base = N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT
+    counts = {}
+    for B, R, r in ((256, 8, 12), (512, 8, 13)):
+        for flags in (base, base | N.HGM_PLAN_LOOKAHEAD):
+            tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=3,
+                                   kernel=N.HGM_KERNEL_REG, planner=flags)
+            counts[(r, flags)] = len(hq.Plan(n, ops, tun))
+        assert counts[(r, base | N.HGM_PLAN_LOOKAHEAD)] < counts[(r, base)], counts
+    assert counts[(12, base | N.HGM_PLAN_LOOKAHEAD)] <= 11

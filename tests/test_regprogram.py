@@ -112,7 +112,8 @@ SHAPES = [(7, 4, 2, 4, 1), (9, 8, 4, 6, 2), (10, 16, 4, 7, 2), (10, 4, 8, 6, 0),
 
 
 @pytest.mark.parametrize("planner", [N.HGM_PLAN_GREEDY, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT,
-                                     N.HGM_PLAN_GREEDY | N.HGM_PLAN_PIVOT | N.HGM_PLAN_BALANCE])
+                                     N.HGM_PLAN_GREEDY | N.HGM_PLAN_PIVOT | N.HGM_PLAN_BALANCE,
+                                     N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT | N.HGM_PLAN_LOOKAHEAD])
 @pytest.mark.parametrize("n,B,R,r,c", SHAPES)
 def test_register_program_matches_oracle(n, B, R, r, c, planner):
     rng = np.random.default_rng(n * 7 + r)
