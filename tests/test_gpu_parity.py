@@ -324,7 +324,8 @@ def test_jit_kernel_parity(dtype, tol):
                                 (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_PIVOT),
                                 (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT),
                                 (64, 8, 10, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT |
-                                 N.HGM_PLAN_BALANCE)]:
+                                 N.HGM_PLAN_BALANCE),
+                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_PIVOT | N.HGM_PLAN_LOOKAHEAD)]:
         tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c,
                                kernel=N.HGM_KERNEL_JIT, ops_block=60, planner=planner)
         plan = hq.Plan(n, ops, tun, 1, dtype)
