@@ -171,3 +171,39 @@ def test_register_program_keeps_loads_coalesced():
     # 600 dense U3 over 8-10 passes: every phase holds 3 register bits (A = 16), so about 600 / 3 / 2
     # phases are needed at the least; a regression to one phase per gate would exceed this bound
     assert phases <= 100
+
+
+FLAG_SETS = [N.HGM_PLAN_GREEDY, N.HGM_PLAN_WINDOW, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE,
+             N.HGM_PLAN_GREEDY | N.HGM_PLAN_FOLD_PHASES, N.HGM_PLAN_GREEDY | N.HGM_PLAN_PIVOT,
+             N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT | N.HGM_PLAN_BALANCE | N.HGM_PLAN_LOOKAHEAD,
+             N.HGM_PLAN_WINDOW | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT]
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_shapes_and_planner_flags(seed):
+    """Random circuits (controls inside and outside the tile, diagonal, anti-
+    diagonal and dense gates, repeated iterations), random valid tile shapes
+    and every combination of planner flags: the register programs reproduce
+    the oracle within 1e-12."""
+    rng = np.random.default_rng(9000 + seed)
+    n = int(rng.integers(5, 11))
+    t, cm, cv, mats = _random_gate_list(rng, n, int(rng.integers(20, 70)))
+    ops = hq.make_ops(t, cm, cv, mats)
+    iterations = int(rng.integers(1, 4))
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+    want = oracle_run_gates(n, list(t) * iterations, list(cm) * iterations, list(cv) * iterations,
+                            list(mats) * iterations, state=psi0)
+    for flags in FLAG_SETS:
+        while True:
+            r = int(rng.integers(3, min(n, 9) + 1))
+            R = 1 << int(rng.integers(0, min(4, r - 1) + 1))
+            B = (1 << r) // (2 * R)
+            c = int(rng.integers(0, min(r - 1, 4) + 1)) if n > r else int(rng.integers(0, r + 1))
+            if B >= 1 and 2 * R <= 32:
+                break
+        tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c,
+                               kernel=N.HGM_KERNEL_REG, planner=flags, ops_block=int(rng.integers(3, 40)))
+        plan = hq.Plan(n, ops, tun, iterations)
+        got = emulate(n, plan, psi0)
+        assert np.abs(got - want).max() < 1e-12, (flags, B, R, r, c)
