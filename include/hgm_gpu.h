@@ -182,6 +182,15 @@ int hgm_state_upload(hgm_state *s, const void *host);
 /* Device-to-device copy between states of equal size and dtype (one streaming kernel). */
 int hgm_state_copy(hgm_state *dst, const hgm_state *src);
 int hgm_state_download(const hgm_state *s, void *host);
+/* Region copies of a host state of host_qubits qubits (same dtype as s):
+ * amplitude j of s is host amplitude region_id | deposit(j, region_mask),
+ * deposit placing the bits of j, lowest first, on the set bits of region_mask.
+ * region_mask holds exactly hgm_state_qubits(s) bits of the host state and
+ * region_id none of them. Gather fills s, scatter writes s back; the other
+ * host amplitudes are not touched. The region projection of t_HYBRID
+ * (ProjectState / GetState, src/core/kernel.cu:321-435) in one pipelined copy. */
+int hgm_state_gather(hgm_state *s, const void *host, int host_qubits, uint64_t region_id, uint64_t region_mask);
+int hgm_state_scatter(const hgm_state *s, void *host, int host_qubits, uint64_t region_id, uint64_t region_mask);
 /* Apply a gate list `iterations` times (plans, uploads the op tables once, launches). */
 int hgm_state_apply(hgm_state *s, const hgm_op *ops, int64_t n_ops, const hgm_tuning *tuning,
                     int iterations, hgm_stats *stats);
@@ -398,15 +407,28 @@ bool setDevice(int device_id);
 
 /* replaces GpuExecutionWrapper (src/core/kernel.cu:303-316; include/core/dgm.h:30).
  * Same argument order. Runs the NULL-terminated PT list on the FP32 host state
- * in place and returns it. gpu_count must be 1. OPS_BLOCK comes from the
+ * in place and returns it. With read_memory == NULL it runs on the region that
+ * ProjectState placed on the device (qubits must be that region's size) and
+ * returns NULL, as src/core/kernel.cu:178-187,285-290 do for t_HYBRID. A gate fires
+ * where (index & control_mask) == control_value on the pair's first amplitude,
+ * the test of src/core/kernel.cu:102-108 with the arguments of
+ * src/core/common.cpp:43-66: a value bit outside the mask (or a 1 asked of
+ * the target bit or of a bit beyond the state) never fires, and a zero mask
+ * ignores the value. gpu_count must be 1. OPS_BLOCK comes from the
  * HGM_OPS_BLOCK environment variable (default 300). Errors print and exit(1). */
 hgm_cfloat *GpuExecutionWrapper(hgm_cfloat *read_memory, hgm_ref_pt **pts, int qubits, int coalesced_bits,
                                 int gpu_region_bits, int gpu_count, int block_size, int repeat_count,
                                 int iterations);
 
-/* replace ProjectState / GetState (src/core/kernel.cu:321,384; include/core/dgm.h:31-32).
- * t_HYBRID is outside this library: both print a message and return false,
- * as src/core/kernel_stub.cpp:27-33 does. */
+/* replace ProjectState / GetState (src/core/kernel.cu:321-435; include/core/dgm.h:31-32),
+ * the GPU side of t_HYBRID (src/core/dgm_par_exec.cpp:447-451). ProjectState
+ * allocates an FP32 device state of 2^region_size amplitudes and gathers into
+ * it the host amplitudes region_id | deposit(j, m), where m is the lowest
+ * region_size set bits of region_mask inside the state (the reference's
+ * portions of 2^coalesced_bits amplitudes, coalesced_bits being the low run of
+ * region_mask); GetState scatters it back and frees it. gpu_count must be 1.
+ * Both return true; bad arguments print and exit(1), as the reference's
+ * error() does. Calls from one thread at a time, as t_HYBRID makes them. */
 bool ProjectState(hgm_cfloat *state, int qubits, int region_size, long region_id, long region_mask, int gpu_count);
 bool GetState(hgm_cfloat *state, int qubits, int region_size, long region_id, long region_mask, int gpu_count);
 

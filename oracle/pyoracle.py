@@ -19,14 +19,19 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libhgmref.so")
+# the same reference engine with its GPU stub left out and this repository's
+# libhgm.so linked in its place (oracle/Makefile, target ref-hgm)
+REF_HGM_SO = os.path.join(HERE, "_ref", "libhgmref_hgm.so")
 
 CAT_DOUBLE = 0
 CAT_REFERENCE = 1
 T_CPU = 0
 T_PAR_CPU = 1
+T_GPU = 2
+T_HYBRID = 3
 
 _lib = None
-_ref = None
+_ref = {}
 
 
 def build(ref_tree: str | None = None) -> None:
@@ -165,14 +170,13 @@ def run(qubits, function, *, gates=None, basis=0, precision=64, iterations=1,
 # --------------------------------------------------------------------------
 # the reference's own CPU engines (oracle/_ref/libhgmref.so)
 # --------------------------------------------------------------------------
-def ref_available() -> bool:
-    return os.path.exists(REF_SO)
+def ref_available(so: str = REF_SO) -> bool:
+    return os.path.exists(so)
 
 
-def _load_ref():
-    global _ref
-    if _ref is None:
-        lib = C.CDLL(REF_SO)
+def _load_ref(so: str = REF_SO):
+    if so not in _ref:
+        lib = C.CDLL(so)
         lib.ref_new.restype = C.c_void_p
         lib.ref_new.argtypes = [C.c_int]
         lib.ref_free.argtypes = [C.c_void_p]
@@ -197,15 +201,16 @@ def _load_ref():
         lib.ref_mul_inv.restype = C.c_long
         lib.ref_revert_bits.argtypes = [C.c_int, C.c_int]
         lib.ref_shor.argtypes = [C.c_long, C.POINTER(C.c_int)]
-        _ref = lib
-    return _ref
+        _ref[so] = lib
+    return _ref[so]
 
 
 class RefDGM:
-    """The reference's DGM (FP32), driven through oracle/ref_driver.cpp."""
+    """The reference's DGM (FP32), driven through oracle/ref_driver.cpp. so picks
+    the build: REF_SO (GPU stub) or REF_HGM_SO (libhgm.so behind t_GPU and t_HYBRID)."""
 
-    def __init__(self, qubits: int):
-        self.lib = _load_ref()
+    def __init__(self, qubits: int, so: str = REF_SO):
+        self.lib = _load_ref(so)
         self.h = self.lib.ref_new(qubits)
         self.qubits = qubits
 

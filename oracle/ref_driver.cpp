@@ -6,7 +6,8 @@
 // tests and the bench's cpu_baseline leg run the reference's real t_CPU and
 // t_PAR_CPU engines and its real circuit generators, in FP32 as the reference
 // computes. The t_GPU path (kernel.cu) is not built: the stub that the
-// reference's own "GPU=stub" build uses stands in for it.
+// reference's own "GPU=stub" build uses stands in for it in libhgmref.so, and
+// this repository's libhgm.so in libhgmref_hgm.so.
 
 #include "core/dgm.h"
 #include "core/gates.h"
@@ -115,7 +116,8 @@ int ref_get_state(void *p, float *out) {
     return 0;
 }
 
-// exec_type 0 = t_CPU, 1 = t_PAR_CPU (the GPU types reach the stub).
+// exec_type 0 = t_CPU, 1 = t_PAR_CPU; 2 = t_GPU and 3 = t_HYBRID reach the GPU
+// stub in libhgmref.so and libhgm.so in libhgmref_hgm.so.
 int ref_execute(void *p, int exec_type, int threads, int cpu_coalesced_bits, int cpu_region_bits, int iterations) {
     DGM *d = static_cast<DGM *>(p);
     d->exec_type = exec_type;

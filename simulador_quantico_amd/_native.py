@@ -27,6 +27,8 @@ HGM_KERNEL_LDS = 1
 HGM_KERNEL_REG = 2
 HGM_KERNEL_JIT = 3
 
+HGM_E_ARG, HGM_E_TUNING, HGM_E_HIP, HGM_E_NOMEM, HGM_E_PARSE, HGM_E_STATE, HGM_E_UNSUPPORTED = range(1, 8)
+
 ERROR_NAMES = {1: "HGM_E_ARG", 2: "HGM_E_TUNING", 3: "HGM_E_HIP", 4: "HGM_E_NOMEM", 5: "HGM_E_PARSE",
                6: "HGM_E_STATE", 7: "HGM_E_UNSUPPORTED"}
 
@@ -100,6 +102,8 @@ SIGNATURES = [
     ("hgm_state_set_basis", C.c_int, [_P, C.c_uint64]),
     ("hgm_state_upload", C.c_int, [_P, _P]),
     ("hgm_state_download", C.c_int, [_P, _P]),
+    ("hgm_state_gather", C.c_int, [_P, _P, C.c_int, C.c_uint64, C.c_uint64]),
+    ("hgm_state_scatter", C.c_int, [_P, _P, C.c_int, C.c_uint64, C.c_uint64]),
     ("hgm_state_copy", C.c_int, [_P, _P]),
     ("hgm_state_apply", C.c_int, [_P, C.POINTER(Op), C.c_int64, C.POINTER(Tuning), C.c_int, C.POINTER(Stats)]),
     ("hgm_state_norm2", C.c_int, [_P, C.POINTER(C.c_double)]),

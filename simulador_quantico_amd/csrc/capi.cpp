@@ -6,6 +6,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "hgm_dgm.hpp"
 #include "hgm_gpu.h"
@@ -332,7 +335,76 @@ int hgm_generate(hgm_dgm *d, const char *which, const int64_t *a, int na, const 
     });
 }
 
+}  // extern "C"
+
 // ------------------------------------------------------------------ reference ABI
+
+namespace {
+
+[[noreturn]] void ref_die(const std::string &msg) {
+    std::printf("error: %s\n", msg.c_str());
+    std::exit(1);
+}
+
+// One PT as the reference's t_GPU kernel applies it: the pair (pos0, pos1)
+// is updated when (pos0 & control_mask) == control_value, pos0 having the
+// target bit clear and no bit at or above qubits (src/core/kernel.cu:102-108,
+// with the control arguments of src/core/common.cpp:53-65, which zero the
+// value when the mask is 0). Returns false for a gate that can never fire.
+// Targets outside the state pass through for the planner to reject.
+bool ref_op(const hgm_ref_pt &p, int qubits, hgm_op *o) {
+    std::memset(o, 0, sizeof(*o));
+    o->target_bit = p.target_bit;
+    uint64_t mask = (uint64_t)p.control_mask, value = mask ? (uint64_t)p.control_value : 0;
+    if (p.target_bit >= 0 && p.target_bit < qubits && qubits < 64) {
+        if (value & ~mask) return false;
+        const uint64_t clear = (uint64_t(1) << p.target_bit) | ~((uint64_t(1) << qubits) - 1);
+        if (value & clear) return false;
+        mask &= ~clear;
+    }
+    o->control_mask = mask;
+    o->control_value = value;
+    for (int k = 0; k < 4; ++k) {
+        o->m[2 * k] = __real__ p.matrix[k];
+        o->m[2 * k + 1] = __imag__ p.matrix[k];
+    }
+    return true;
+}
+
+// The region that ProjectState placed on the device, kept for
+// GpuExecutionWrapper(NULL, ...) and GetState (the reference's gpu_mem,
+// src/core/kernel.cu:321-435).
+std::mutex g_proj_mu;
+hgm_state *g_proj = nullptr;
+
+// Checks the arguments of ProjectState / GetState and returns the region's
+// bits: the lowest region_size set bits of region_mask inside the state. The
+// reference copies 2^(region_size - c) portions of 2^c amplitudes, c being the
+// low run of region_mask counted inside the state, at the subsets of the mask
+// bits above c taken in increasing order (src/core/kernel.cu:329-357), which
+// are the same amplitudes.
+uint64_t ref_region(const hgm_cfloat *state, int qubits, int region_size, long region_id, long region_mask,
+                    int gpu_count, const char *who) {
+    const std::string w(who);
+    if (gpu_count != 1) ref_die(w + ": gpu_count " + std::to_string(gpu_count) + ": this library drives one GPU per process");
+    if (!state || qubits < 1 || qubits > 40 || region_size < 1 || region_size > qubits)
+        ref_die(w + ": bad state, qubits or region_size");
+    const uint64_t all = (uint64_t(1) << qubits) - 1;
+    uint64_t bits = (uint64_t)region_mask & all, mask = 0;
+    for (int k = 0; k < region_size && bits; ++k) {
+        mask |= bits & (~bits + 1);
+        bits &= bits - 1;
+    }
+    if (__builtin_popcountll(mask) != region_size)
+        ref_die(w + ": region_mask has fewer than region_size bits inside the state");
+    if (((uint64_t)region_id & ~all) || ((uint64_t)region_id & mask))
+        ref_die(w + ": region_id outside the state or on a region bit");
+    return mask;
+}
+
+}  // namespace
+
+extern "C" {
 
 bool setDevice(int device_id) { return hgm_init(device_id) != HGM_OK; }
 
@@ -341,23 +413,11 @@ hgm_cfloat *GpuExecutionWrapper(hgm_cfloat *read_memory, hgm_ref_pt **pts, int q
                                 int iterations) {
     static const bool debug = std::getenv("HYBRID_DEBUG") != nullptr;
     auto t0 = std::chrono::steady_clock::now();
-    if (gpu_count != 1) {
-        std::printf("error: gpu_count %d: this library drives one GPU per process\n", gpu_count);
-        std::exit(1);
-    }
+    if (gpu_count != 1) ref_die("gpu_count " + std::to_string(gpu_count) + ": this library drives one GPU per process");
     std::vector<hgm_op> ops;
     for (size_t i = 0; pts && pts[i]; ++i) {
-        const hgm_ref_pt *p = pts[i];
         hgm_op o;
-        std::memset(&o, 0, sizeof(o));
-        o.target_bit = p->target_bit;
-        o.control_mask = (uint64_t)p->control_mask;
-        o.control_value = (uint64_t)p->control_value;
-        for (int k = 0; k < 4; ++k) {
-            o.m[2 * k] = __real__ p->matrix[k];
-            o.m[2 * k + 1] = __imag__ p->matrix[k];
-        }
-        ops.push_back(o);
+        if (ref_op(*pts[i], qubits, &o)) ops.push_back(o);
     }
     hgm_tuning t;
     hgm_default_tuning(&t);
@@ -368,7 +428,17 @@ hgm_cfloat *GpuExecutionWrapper(hgm_cfloat *read_memory, hgm_ref_pt **pts, int q
     if (const char *e = std::getenv("HGM_OPS_BLOCK")) t.ops_block = std::atoi(e);
     hgm_stats st;
     std::memset(&st, 0, sizeof(st));
-    int rc = hgm_execute(read_memory, qubits, HGM_C64, ops.data(), (int64_t)ops.size(), &t, iterations, &st);
+    int rc;
+    if (read_memory) {
+        rc = hgm_execute(read_memory, qubits, HGM_C64, ops.data(), (int64_t)ops.size(), &t, iterations, &st);
+    } else {
+        std::lock_guard<std::mutex> g(g_proj_mu);
+        if (!g_proj) ref_die("GpuExecutionWrapper(NULL, ...): no state was projected (ProjectState)");
+        if (hgm_state_qubits(g_proj) != qubits)
+            ref_die("GpuExecutionWrapper(NULL, ...): qubits " + std::to_string(qubits) + " but the projected region has " +
+                    std::to_string(hgm_state_qubits(g_proj)));
+        rc = hgm_state_apply(g_proj, ops.data(), (int64_t)ops.size(), &t, iterations, &st);
+    }
     if (rc) {
         std::printf("error: %d - %s\n", rc, hgm_last_error());
         std::exit(1);
@@ -380,14 +450,47 @@ hgm_cfloat *GpuExecutionWrapper(hgm_cfloat *read_memory, hgm_ref_pt **pts, int q
     return read_memory;
 }
 
-bool ProjectState(hgm_cfloat *, int, int, long, long, int) {
-    std::fprintf(stderr, "hgm: ProjectState belongs to t_HYBRID, which this library does not provide.\n");
-    return false;
+bool ProjectState(hgm_cfloat *state, int qubits, int region_size, long region_id, long region_mask, int gpu_count) {
+    static const bool debug = std::getenv("HYBRID_DEBUG") != nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    const uint64_t mask = ref_region(state, qubits, region_size, region_id, region_mask, gpu_count, "ProjectState");
+    std::lock_guard<std::mutex> g(g_proj_mu);
+    if (g_proj) {   // a region never collected by GetState: the reference overwrites gpu_mem
+        hgm_state_destroy(g_proj);
+        g_proj = nullptr;
+    }
+    int rc = hgm_state_create(region_size, HGM_C64, &g_proj);
+    if (!rc) rc = hgm_state_gather(g_proj, state, qubits, (uint64_t)region_id, mask);
+    if (rc) {
+        std::printf("error: %d - %s\n", rc, hgm_last_error());
+        std::exit(1);
+    }
+    if (debug)
+        std::fprintf(stderr, "[PROJSTATE] region_size=%d coalesced_bits=%d elapsed_ms=%.3f\n", region_size,
+                     __builtin_ctzll(~mask),
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    return true;
 }
 
-bool GetState(hgm_cfloat *, int, int, long, long, int) {
-    std::fprintf(stderr, "hgm: GetState belongs to t_HYBRID, which this library does not provide.\n");
-    return false;
+bool GetState(hgm_cfloat *state, int qubits, int region_size, long region_id, long region_mask, int gpu_count) {
+    static const bool debug = std::getenv("HYBRID_DEBUG") != nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    const uint64_t mask = ref_region(state, qubits, region_size, region_id, region_mask, gpu_count, "GetState");
+    std::lock_guard<std::mutex> g(g_proj_mu);
+    if (!g_proj || hgm_state_qubits(g_proj) != region_size)
+        ref_die("GetState: no projected region of " + std::to_string(region_size) + " qubits (ProjectState)");
+    int rc = hgm_state_scatter(g_proj, state, qubits, (uint64_t)region_id, mask);
+    hgm_state_destroy(g_proj);
+    g_proj = nullptr;
+    if (rc) {
+        std::printf("error: %d - %s\n", rc, hgm_last_error());
+        std::exit(1);
+    }
+    if (debug)
+        std::fprintf(stderr, "[GETSTATE]  region_size=%d coalesced_bits=%d elapsed_ms=%.3f\n", region_size,
+                     __builtin_ctzll(~mask),
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    return true;
 }
 
 }  // extern "C"

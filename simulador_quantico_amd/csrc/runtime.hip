@@ -81,6 +81,8 @@ struct Ctx {
     uint64_t *pidx = nullptr;
     double *h_partial = nullptr;  // pinned host mirror
     uint64_t *h_pidx = nullptr;
+    char *h_stage[2] = {nullptr, nullptr};          // pinned chunks of the region copies
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 struct JitModule {
@@ -514,6 +516,10 @@ int hgm_release_cache(void) {
     drop(&c.rops, &c.rops_cap);
     drop(&c.maps, &c.maps_cap);
     drop(&c.mems, &c.mems_cap);
+    for (char *&h : c.h_stage) {
+        if (h) hipHostFree(h);
+        h = nullptr;
+    }
     for (auto &kv : jit_modules()) hipModuleUnload(kv.second.mod);
     jit_modules().clear();
     return HGM_OK;
@@ -618,6 +624,122 @@ int hgm_state_download(const hgm_state *s, void *host) {
     std::lock_guard<std::mutex> g(c.mu);
     HGM_HIP(hipMemcpyAsync(host, s->d, s->bytes, hipMemcpyDeviceToHost, c.stream), "download");
     HGM_HIP(hipStreamSynchronize(c.stream), "download");
+    return HGM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Region copies between a host state and a smaller device state. Device
+// amplitude j is host amplitude region_id | deposit(j, region_mask). The low
+// run of region_mask makes portions of 2^run consecutive amplitudes (the
+// reference's coalesced bits, src/core/kernel.cu:330-343); the portions are
+// visited in device order by the subset walk base -> ((base | ~hi) + 1) & hi
+// over the remaining mask bits. The host side is pageable memory, so the bytes
+// go through two pinned chunks: the host copy into one chunk overlaps the DMA
+// of the other.
+constexpr size_t kStageBytes = size_t(16) << 20;
+
+struct RegionWalk {
+    const uint64_t id, hi;   // fixed bits; region bits above the run, shifted down by run
+    const int run;
+    const size_t amp, portion;
+    uint64_t base = 0;       // deposited portion index
+    size_t within = 0;       // byte offset inside the current portion
+    RegionWalk(uint64_t region_id, uint64_t mask, size_t amp_bytes)
+        : id(region_id), hi(mask >> __builtin_ctzll(~mask)), run(__builtin_ctzll(~mask)), amp(amp_bytes),
+          portion(amp_bytes << __builtin_ctzll(~mask)) {}
+    // the next n bytes in device order, as host runs: fn(host_byte_offset, bytes, chunk_offset)
+    template <class F>
+    void next(size_t n, F fn) {
+        for (size_t w = 0; w < n;) {
+            const size_t k = std::min(portion - within, n - w);
+            fn(((base << run) | id) * amp + within, k, w);
+            w += k;
+            within += k;
+            if (within == portion) {
+                within = 0;
+                base = ((base | ~hi) + 1) & hi;
+            }
+        }
+    }
+};
+
+int region_check(const hgm_state *s, const void *host, int host_qubits, uint64_t region_id, uint64_t region_mask) {
+    if (!s || !host) { t_err = "null argument"; return HGM_E_ARG; }
+    if (host_qubits < s->qubits || host_qubits > 40) { t_err = "region copy: host state smaller than the region"; return HGM_E_ARG; }
+    const uint64_t all = (uint64_t(1) << host_qubits) - 1;
+    if ((region_mask & ~all) || (region_id & ~all) || (region_id & region_mask) ||
+        __builtin_popcountll(region_mask) != s->qubits) {
+        t_err = "region copy: region_mask must hold " + std::to_string(s->qubits) +
+                " bits of the state and region_id none of them";
+        return HGM_E_ARG;
+    }
+    return HGM_OK;
+}
+
+int stage_locked() {
+    Ctx &c = ctx();
+    for (int i = 0; i < 2; ++i) {
+        if (!c.stage_ev[i]) HGM_HIP(hipEventCreateWithFlags(&c.stage_ev[i], hipEventDisableTiming), "hipEventCreate");
+        if (!c.h_stage[i])
+            HGM_HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_stage[i]), kStageBytes, hipHostMallocDefault),
+                    "hipHostMalloc(staging)");
+    }
+    return HGM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hgm_state_gather(hgm_state *s, const void *host, int host_qubits, uint64_t region_id, uint64_t region_mask) {
+    int rc = region_check(s, host, host_qubits, region_id, region_mask);
+    if (rc) return rc;
+    Ctx &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if ((rc = stage_locked())) return rc;
+    const char *h = static_cast<const char *>(host);
+    char *d = static_cast<char *>(s->d);
+    const size_t chunk = std::min(kStageBytes, s->bytes);   // both powers of two: chunks tile the state
+    RegionWalk walk(region_id, region_mask, amp_bytes(s->dtype));
+    for (size_t off = 0, i = 0; off < s->bytes; off += chunk, ++i) {
+        char *buf = c.h_stage[i & 1];
+        if (i >= 2) HGM_HIP(hipEventSynchronize(c.stage_ev[i & 1]), "region gather");
+        walk.next(chunk, [&](size_t src, size_t n, size_t at) { std::memcpy(buf + at, h + src, n); });
+        HGM_HIP(hipMemcpyAsync(d + off, buf, chunk, hipMemcpyHostToDevice, c.stream), "region gather");
+        HGM_HIP(hipEventRecord(c.stage_ev[i & 1], c.stream), "hipEventRecord");
+    }
+    HGM_HIP(hipStreamSynchronize(c.stream), "region gather");
+    return HGM_OK;
+}
+
+int hgm_state_scatter(const hgm_state *s, void *host, int host_qubits, uint64_t region_id, uint64_t region_mask) {
+    int rc = region_check(s, host, host_qubits, region_id, region_mask);
+    if (rc) return rc;
+    Ctx &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if ((rc = stage_locked())) return rc;
+    char *h = static_cast<char *>(host);
+    const char *d = static_cast<const char *>(s->d);
+    const size_t chunk = std::min(kStageBytes, s->bytes);
+    const size_t n_chunks = s->bytes / chunk;
+    auto fetch = [&](size_t i) -> int {
+        HGM_HIP(hipMemcpyAsync(c.h_stage[i & 1], d + i * chunk, chunk, hipMemcpyDeviceToHost, c.stream),
+                "region scatter");
+        HGM_HIP(hipEventRecord(c.stage_ev[i & 1], c.stream), "hipEventRecord");
+        return HGM_OK;
+    };
+    for (size_t i = 0; i < std::min<size_t>(2, n_chunks); ++i)
+        if ((rc = fetch(i))) return rc;
+    RegionWalk walk(region_id, region_mask, amp_bytes(s->dtype));
+    for (size_t i = 0; i < n_chunks; ++i) {
+        const char *buf = c.h_stage[i & 1];
+        HGM_HIP(hipEventSynchronize(c.stage_ev[i & 1]), "region scatter");
+        walk.next(chunk, [&](size_t dst, size_t n, size_t at) { std::memcpy(h + dst, buf + at, n); });
+        if (i + 2 < n_chunks && (rc = fetch(i + 2))) return rc;
+    }
     return HGM_OK;
 }
 

@@ -231,6 +231,26 @@ class State:
         check(lib.hgm_state_download(self._h, out.ctypes.data))
         return out
 
+    def _region_host(self, host) -> int:
+        if not (isinstance(host, np.ndarray) and host.dtype == self.dtype and host.flags.c_contiguous):
+            raise HGMError(N.HGM_E_ARG, f"host state must be a contiguous {self.dtype} array")
+        n = int(host.size).bit_length() - 1
+        if host.size != 1 << n:
+            raise HGMError(N.HGM_E_ARG, "host state size is not a power of two")
+        return n
+
+    def gather(self, host: np.ndarray, region_id: int, region_mask: int) -> None:
+        """Fill this state with the host amplitudes region_id | deposit(j, region_mask)
+        (the region projection of src/core/kernel.cu:321-382)."""
+        n = self._region_host(host)
+        check(lib.hgm_state_gather(self._h, host.ctypes.data, n, int(region_id), int(region_mask)))
+
+    def scatter(self, host: np.ndarray, region_id: int, region_mask: int) -> None:
+        """Write this state back to the host amplitudes region_id | deposit(j, region_mask)
+        (src/core/kernel.cu:384-435); the other host amplitudes are untouched."""
+        n = self._region_host(host)
+        check(lib.hgm_state_scatter(self._h, host.ctypes.data, n, int(region_id), int(region_mask)))
+
     def apply(self, ops, tuning=None, iterations: int = 1) -> dict:
         st = Stats()
         t = tuning if tuning is not None else N.default_tuning()

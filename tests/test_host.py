@@ -222,10 +222,31 @@ def test_circuit_width_must_match_state():
         d.execute(1)
 
 
-def test_ref_stubs_report_hybrid_unsupported(capfd):
-    assert N.lib.ProjectState(None, 4, 2, 0, 0, 1) is False
-    assert N.lib.GetState(None, 4, 2, 0, 0, 1) is False
-    assert "t_HYBRID" in capfd.readouterr().err
+@pytest.mark.parametrize("call,message", [
+    ("ProjectState(None, 4, 2, 0, 3, 1)", "ProjectState: bad state"),
+    ("ProjectState(buf, 4, 2, 0, 3, 2)", "gpu_count 2"),
+    ("ProjectState(buf, 4, 3, 0, 0b10011, 1)", "fewer than region_size bits"),
+    ("ProjectState(buf, 4, 2, 0b0001, 0b0011, 1)", "region_id outside the state or on a region bit"),
+    ("GetState(buf, 4, 2, 1 << 4, 0b0011, 1)", "region_id outside the state or on a region bit"),
+    ("GetState(buf, 4, 5, 0, 0b1111, 1)", "GetState: bad state"),
+])
+def test_projection_argument_errors_exit(call, message):
+    """ProjectState / GetState check their arguments before any device call and,
+    like the reference's error(), print and exit(1) (run in a child process)."""
+    import subprocess
+    import sys
+    code = ("import numpy as np; from simulador_quantico_amd import _native as N; "
+            "buf = np.zeros(16, np.complex64).ctypes.data; N.lib." + call)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert r.returncode == 1
+    assert message in r.stdout
+
+
+def test_region_copy_null_arguments():
+    host = np.zeros(16, np.complex64)
+    assert N.lib.hgm_state_gather(None, host.ctypes.data, 4, 0, 0b11) == N.HGM_E_ARG
+    assert N.lib.hgm_state_scatter(None, host.ctypes.data, 4, 0, 0b11) == N.HGM_E_ARG
 
 
 def test_measure_host_path_matches_reference(golden):
