@@ -9,6 +9,7 @@ Bars: region copies are bit-exact (they move bytes); FP32 states against the
 FP64 oracle or the reference's FP32 t_CPU within 2e-5 max-abs."""
 import ctypes as C
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -90,10 +91,20 @@ def test_region_gather_scatter_bit_exact(n, mask, rid, dtype):
     s = hq.State(k, dtype)
     s.gather(host, rid, mask)
     assert np.array_equal(s.download(), host[idx])
+    t0 = time.perf_counter()
+    s.gather(host, rid, mask)              # timed again, staging set up
+    t1 = time.perf_counter()
     y = random_host(rng, k, dtype)
+    tu = time.perf_counter()
     s.upload(y)
+    tu = time.perf_counter() - tu
     out = host.copy()
+    t2 = time.perf_counter()
     s.scatter(out, rid, mask)
+    t3 = time.perf_counter()
+    nbytes = idx.size * np.dtype(dtype).itemsize
+    say(f"  {nbytes / 2**20:.1f} MiB region: gather {nbytes / (t1 - t0) / 1e9:.1f} GB/s, "
+        f"scatter {nbytes / (t3 - t2) / 1e9:.1f} GB/s, contiguous upload {nbytes / tu / 1e9:.1f} GB/s")
     want = host.copy()
     want[idx] = y
     assert np.array_equal(out, want)
