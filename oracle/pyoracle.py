@@ -201,6 +201,10 @@ def _load_ref(so: str = REF_SO):
         lib.ref_mul_inv.restype = C.c_long
         lib.ref_revert_bits.argtypes = [C.c_int, C.c_int]
         lib.ref_shor.argtypes = [C.c_long, C.POINTER(C.c_int)]
+        lib.ref_general_hadamard.argtypes = [C.c_long, C.c_int, C.c_int]
+        lib.ref_grover_run.argtypes = [C.c_long, C.c_long, C.c_int, C.c_int, C.c_ulong]
+        lib.ref_grover_run.restype = C.c_long
+        lib.ref_shor_run.argtypes = [C.c_long, C.c_int, C.c_int, C.c_ulong, C.POINTER(C.c_int)]
         _ref[so] = lib
     return _ref[so]
 

@@ -11,9 +11,11 @@
 
 #include "core/dgm.h"
 #include "core/gates.h"
+#include "algorithms/lib_general.h"
 #include "algorithms/lib_grover.h"
 #include "algorithms/lib_shor.h"
 
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -176,6 +178,29 @@ int ref_revert_bits(int v, int bits) { return revert_bits(v, bits); }
 // or {0, 0}.
 void ref_shor(long number, int *out) {
     std::vector<int> f = Shor(number, t_CPU, 1, 14, 11, 1, 8, 4, 64, 2);
+    out[0] = f.size() == 2 ? f[0] : 0;
+    out[1] = f.size() == 2 ? f[1] : 0;
+}
+
+// The reference's application drivers with any exec_type and the CLI's tuning
+// defaults (include/cli/cli_common.h:4-13): general.out's HadamardNQubits
+// (src/algorithms/lib_general.cpp:15-47, 3 iterations, src/cli/general.cpp:43)
+// prints amplitude 0 to stdout; Grover (src/algorithms/lib_grover.cpp:15-91)
+// returns the measured value; Shor (src/algorithms/lib_shor.cpp:18-163) the
+// factors, out = {f, N/f} or {0, 0}. The seed goes to the measurement RNG.
+void ref_general_hadamard(long qubits, int exec_type, int threads) {
+    HadamardNQubits(qubits, 3, exec_type, threads, 14, 11, 1, 8, 4, 64, 2);
+    std::fflush(stdout);
+}
+
+long ref_grover_run(long qubits, long search_value, int exec_type, int threads, unsigned long seed) {
+    seed_rng(seed);
+    return Grover(qubits, search_value, exec_type, threads, 14, 11, 1, 8, 4, 64, 2);
+}
+
+void ref_shor_run(long number, int exec_type, int threads, unsigned long seed, int *out) {
+    seed_rng(seed);
+    std::vector<int> f = Shor(number, exec_type, threads, 14, 11, 1, 8, 4, 64, 2);
     out[0] = f.size() == 2 ? f[0] : 0;
     out[1] = f.size() == 2 ? f[1] : 0;
 }

@@ -268,3 +268,36 @@ def test_reference_engine_linked_against_libhgm(n, threads):
         out[exec_type] = r.state()
     assert np.abs(out[po.T_GPU] - out[po.T_CPU]).max() < TOL32
     assert np.abs(out[po.T_HYBRID] - out[po.T_CPU]).max() < TOL32
+
+
+@pytest.mark.skipif(not po.ref_available(po.REF_HGM_SO),
+                    reason="oracle/_ref/libhgmref_hgm.so not built (needs the reference tree)")
+def test_reference_smoke_checks_on_libhgm(capfd):
+    """The GPU half of the reference's own smoke test (tests/smoke_test.sh:86-101,114-118),
+    through the reference's drivers linked against libhgm.so: general.out's
+    uniform amplitude 1/sqrt(2^n) (|error| < 1e-4, :26) under t_GPU at 10/14/18
+    qubits and t_HYBRID at 10/14; Grover at 12 qubits finds search_value 10 in
+    at least 7 of 10 seeded runs; Shor at 15 qubits (N = 57) succeeds at least
+    once in 16 seeded attempts. t_GPU runs with gpu_count 1, t_HYBRID with 2 threads."""
+    lib = po._load_ref(po.REF_HGM_SO)
+    for n, exec_type, threads in [(10, po.T_GPU, 1), (14, po.T_GPU, 1), (18, po.T_GPU, 1),
+                                  (10, po.T_HYBRID, 2), (14, po.T_HYBRID, 2)]:
+        capfd.readouterr()
+        lib.ref_general_hadamard(n, exec_type, threads)
+        line = capfd.readouterr().out.strip().splitlines()[-1]
+        say("general", n, exec_type, line)
+        assert abs(float(line.split()[1]) - 2.0 ** (-n / 2)) < 1e-4
+    for exec_type, threads in [(po.T_GPU, 1), (po.T_HYBRID, 2)]:
+        found = [lib.ref_grover_run(12, 10, exec_type, threads, seed) for seed in range(10)]
+        say("grover", exec_type, found)
+        assert sum(v == 10 for v in found) >= 7
+    out = (C.c_int * 2)()
+    for exec_type, threads in [(po.T_GPU, 1), (po.T_HYBRID, 2)]:
+        tries = []
+        for seed in range(16):
+            lib.ref_shor_run(57, exec_type, threads, seed, out)
+            tries.append(tuple(out))
+            if out[0]:
+                break
+        say("shor", exec_type, tries)
+        assert sorted(tries[-1]) == [3, 19]
