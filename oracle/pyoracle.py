@@ -201,6 +201,7 @@ def _load_ref(so: str = REF_SO):
         lib.ref_mul_inv.restype = C.c_long
         lib.ref_revert_bits.argtypes = [C.c_int, C.c_int]
         lib.ref_shor.argtypes = [C.c_long, C.POINTER(C.c_int)]
+        lib.ref_set_gpu_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         lib.ref_general_hadamard.argtypes = [C.c_long, C.c_int, C.c_int]
         lib.ref_grover_run.argtypes = [C.c_long, C.c_long, C.c_int, C.c_int, C.c_ulong]
         lib.ref_grover_run.restype = C.c_long
@@ -249,6 +250,9 @@ class RefDGM:
         m = np.zeros((n, 8), np.float32)
         self.lib.ref_get_pts(self.h, t.ctypes.data, cm.ctypes.data, cv.ctypes.data, m.ctypes.data)
         return t, cm, cv, m
+
+    def set_gpu_tuning(self, gpu_region_bits, gpu_coalesced_bits, block_size, repeat_count):
+        self.lib.ref_set_gpu_tuning(self.h, gpu_region_bits, gpu_coalesced_bits, block_size, repeat_count)
 
     def set_basis(self, index):
         self.lib.ref_set_basis(self.h, index)

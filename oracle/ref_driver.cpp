@@ -53,6 +53,16 @@ void *ref_new(int qubits) {
 
 void ref_free(void *p) { delete static_cast<DGM *>(p); }
 
+// GPU tuning of t_GPU / t_HYBRID (validated by DGM::validateTuning,
+// src/core/dgm_core.cpp:177-241, which exits on a bad tuple).
+void ref_set_gpu_tuning(void *p, int gpu_region_bits, int gpu_coalesced_bits, int block_size, int repeat_count) {
+    DGM *d = static_cast<DGM *>(p);
+    d->gpu_region_bits = gpu_region_bits;
+    d->gpu_coalesced_bits = gpu_coalesced_bits;
+    d->block_size = block_size;
+    d->repeat_count = repeat_count;
+}
+
 int ref_add_gate(void *p, const char *name, const float *m8) {
     DGM *d = static_cast<DGM *>(p);
     float complex a[4];

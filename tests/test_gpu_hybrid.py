@@ -301,3 +301,28 @@ def test_reference_smoke_checks_on_libhgm(capfd):
                 break
         say("shor", exec_type, tries)
         assert sorted(tries[-1]) == [3, 19]
+
+
+@pytest.mark.skipif(not po.ref_available(po.REF_HGM_SO),
+                    reason="oracle/_ref/libhgmref_hgm.so not built (needs the reference tree)")
+def test_reference_engine_golden_circuits_on_libhgm(golden):
+    """Every golden circuit through the reference's own DGM with t_GPU and t_HYBRID
+    (2 threads) backed by libhgm.so, against the FP32 states the reference's t_CPU
+    wrote into the fixtures (tests/golden/states.npz). Below 8 qubits the tile is
+    the whole state (2^(n-1) threads of one pair each), since the reference's
+    validateTuning clamps gpu_region_bits to the qubit count and then requires
+    2*block_size*repeat_count == 2^gpu_region_bits."""
+    for c in golden["circuits"]:
+        ref32 = golden["states"][f"{c['name']}__tcpu"]
+        for exec_type, threads in [(po.T_GPU, 1), (po.T_HYBRID, 2)]:
+            say("golden via reference engine", c["name"], exec_type)
+            r = po.RefDGM(c["qubits"], po.REF_HGM_SO)
+            if c["qubits"] < 8:   # the CLI default tile (2^8) is larger than the state: validateTuning exits
+                n = c["qubits"]
+                r.set_gpu_tuning(n, min(4, n), 1 << (n - 1), 1)
+            for name, m in c["gates"].items():
+                r.add_gate(name, [complex(a, b) for a, b in m])
+            r.set_function(c["steps"], c["iterations"])
+            r.set_basis(c["basis"])
+            r.execute(exec_type, threads=threads, coalesced_bits=11, region_bits=14, iterations=1)
+            assert np.abs(r.state() - ref32).max() < TOL32, (c["name"], exec_type)
