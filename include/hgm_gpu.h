@@ -118,9 +118,10 @@ typedef struct hgm_tuning {
     int32_t lookahead;           /* greedy planner scan window in gates (0 = default 4096) */
 } hgm_tuning;
 
-/* Per-call statistics. bytes_alg = sum over passes of 2 * tile amplitudes *
- * sizeof(amplitude) = the HBM bytes one read and one write of every loaded
- * amplitude costs. */
+/* Call statistics. Every call that takes an hgm_stats* adds its figures to
+ * the fields, so zero the struct before the first call to count from there.
+ * bytes_alg = sum over passes of 2 * tile amplitudes * sizeof(amplitude) =
+ * the HBM bytes one read and one write of every loaded amplitude costs. */
 typedef struct hgm_stats {
     int64_t gates;           /* gates applied (iterations included) */
     int64_t passes;          /* kernel launches over the state */

@@ -261,10 +261,9 @@ int hgm_shor(hgm_dgm *cfg, int64_t number, int64_t out[4], hgm_stats *stats) {
         out[1] = r.factors.size() == 2 ? r.factors[1] : 0;
         out[2] = r.base;
         out[3] = r.phase_bits;
-        if (stats) {
-            std::memset(stats, 0, sizeof *stats);
-            stats->gates = r.gates;
-            stats->kernel_ms = r.kernel_ms;
+        if (stats) {   // added, like every other call that takes an hgm_stats*
+            stats->gates += r.gates;
+            stats->kernel_ms += r.kernel_ms;
         }
         return HGM_OK;
     });
