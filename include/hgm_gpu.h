@@ -223,7 +223,7 @@ int hgm_plan_pass(const hgm_plan *p, int64_t i, uint64_t *local_mask, int64_t *n
 int hgm_plan_shape(const hgm_plan *p, int32_t *region_bits, int32_t *block_size, int32_t *repeat_count,
                    int32_t *coalesced_bits);
 int hgm_state_run_plan(hgm_state *s, const hgm_plan *p, hgm_stats *stats);
-/* Kernel family the plan runs on (HGM_KERNEL_LDS or HGM_KERNEL_REG). */
+/* Kernel family the plan runs on (HGM_KERNEL_LDS, HGM_KERNEL_REG or HGM_KERNEL_JIT; never AUTO). */
 int hgm_plan_kernel(const hgm_plan *p);
 /* Register-kernel program of pass i, for inspection: rops receives n_rops
  * records of 104 bytes, local_pos the 16 tile-bit -> state-bit entries, and
