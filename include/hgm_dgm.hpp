@@ -62,6 +62,8 @@ struct PT {
     long control_count = 0;
 
     long matrixType() const;        // DENSE / DIAG_PRI / DIAG_SEC (src/core/common.cpp:33-38)
+    void print() const;             // fields to stdout, as src/core/common.cpp:68-74 prints them
+    void printMatrix() const;       // the 2x2 matrix, as src/core/common.cpp:77-85
 };
 
 // Gate catalog: name -> row-major 2x2 matrix; addGate never overwrites
@@ -121,6 +123,7 @@ public:
     std::map<long, Group> genGroups(const std::string &step);
     void genPTs(std::map<long, Group> &groups, std::vector<PT *> &step_pts);
     void CountOps(int iterations = 1);
+    void printPTs() const;          // print() of every PT (src/core/dgm_core.cpp:35-39)
     void validateTuning();
 
     hgm_tuning tuning() const;

@@ -364,6 +364,9 @@ int hgm_dgm_measure_many(hgm_dgm *d, const int *qubits, int n, int64_t *values, 
 int hgm_dgm_sync_host(hgm_dgm *d);                             /* resident mode: download */
 int hgm_dgm_stats(const hgm_dgm *d, hgm_stats *out);
 int hgm_dgm_count_ops(hgm_dgm *d, int iterations, int64_t counts[7]); /* CountOps, dgm_core.cpp:270-299 */
+/* printPTs (dgm_core.cpp:35-39): every PT's fields to stdout, in PT::print's
+ * order and formats (common.cpp:68-74). */
+int hgm_dgm_print_pts(const hgm_dgm *d);
 void hgm_seed_rng(uint64_t seed);                              /* seed_rng, common.cpp:12-14 */
 /* Classical steps of Shor's algorithm: continued-fraction approximation of
  * num/den with denominator below 2^width (replaces quantum_frac_approx,

@@ -87,6 +87,11 @@ int ref_set_function_steps(void *p, const char *const *steps, int n, int iterati
 
 int ref_qubits(void *p) { return (int)static_cast<DGM *>(p)->qubits; }
 
+void ref_print_pts(void *p) {
+    static_cast<DGM *>(p)->printPTs();
+    std::fflush(stdout);
+}
+
 long ref_num_pts(void *p) {
     DGM *d = static_cast<DGM *>(p);
     long n = 0;

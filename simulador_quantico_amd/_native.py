@@ -155,6 +155,7 @@ SIGNATURES = [
     ("hgm_dgm_sync_host", C.c_int, [_P]),
     ("hgm_dgm_stats", C.c_int, [_P, C.POINTER(Stats)]),
     ("hgm_dgm_count_ops", C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
+    ("hgm_dgm_print_pts", C.c_int, [_P]),
     ("hgm_seed_rng", None, [C.c_uint64]),
     ("hgm_shor", C.c_int, [_P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(Stats)]),
     ("hgm_frac_approx", None, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]),

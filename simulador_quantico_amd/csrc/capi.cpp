@@ -242,6 +242,12 @@ int hgm_dgm_count_ops(hgm_dgm *d, int iterations, int64_t counts[7]) {
     return HGM_OK;
 }
 
+int hgm_dgm_print_pts(const hgm_dgm *d) {
+    if (!d) return HGM_E_ARG;
+    d->core().printPTs();
+    return HGM_OK;
+}
+
 void hgm_seed_rng(uint64_t seed) { hgm::seed_rng((unsigned long)seed); }
 
 void hgm_frac_approx(int *num, int *den, int width) { hgm::quantum_frac_approx(num, den, width); }

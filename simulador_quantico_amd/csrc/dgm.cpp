@@ -193,6 +193,33 @@ void DGMCore::genPTs(std::map<long, Group> &groups, std::vector<PT *> &step_pts)
     }
 }
 
+// Debug printing with the reference's field order and formats
+// (src/core/common.cpp:68-85, src/core/dgm_core.cpp:35-39), so the output of
+// the two builds can be compared line by line.
+void PT::print() const {
+    std::printf("qubits: %d\nmatrix_size: %d\nspan_start_bit: %d\ntarget_bit: %d\nAffect: %d\n", qubits, matrix_size,
+                span_start_bit, target_bit, (int)affected);
+    std::printf("Ctrl-(value: %ld, mask: %ld, count: %ld)\n", control_value, control_mask, control_count);
+    for (long i = 0; i < control_count && i < (long)control_bit_positions.size(); ++i)
+        std::printf("%d: %ld\n", (int)i, control_bit_positions[(size_t)i]);
+}
+
+void PT::printMatrix() const {
+    for (int row = 0; row < matrix_size; ++row) {
+        for (int col = 0; col < matrix_size; ++col) {
+            const cplx v = matrix ? matrix[row * matrix_size + col] : cplx(0, 0);
+            // the reference's matrices are float complex: print the same rounding
+            std::printf("%d: %.4f, %.4f  \t", row * matrix_size + col, (double)(float)v.real(), (double)(float)v.imag());
+        }
+        std::printf("\n");
+    }
+}
+
+void DGMCore::printPTs() const {
+    for (size_t i = 0; pts && pts[i]; ++i) pts[i]->print();
+    std::fflush(stdout);
+}
+
 void DGMCore::CountOps(int iterations) {
     dense = main_diag = sec_diag = c_dense = c_main_diag = c_sec_diag = 0;
     for (size_t i = 0; pts && pts[i]; ++i) {

@@ -158,6 +158,10 @@ class DGM:
         check(lib.hgm_dgm_measure(self._h, int(qubit), C.byref(out)))
         return out.value
 
+    def printPTs(self) -> None:
+        """Every PT's fields to stdout as the reference prints them (src/core/dgm_core.cpp:35-39)."""
+        check(lib.hgm_dgm_print_pts(self._h))
+
     def CountOps(self, iterations: int = 1) -> dict:
         c = (C.c_int64 * 7)()
         check(lib.hgm_dgm_count_ops(self._h, iterations, c))

@@ -344,3 +344,24 @@ def test_gpu_entry_points_fail_cleanly_without_a_device():
     N.lib.hgm_server_default_config(C.byref(bad))
     bad.max_active = 0
     assert N.lib.hgm_server_create(C.byref(bad), C.byref(h)) == 1   # HGM_E_ARG: checked before any device call
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built (needs the reference tree)")
+def test_print_pts_matches_reference(capfd, golden):
+    """printPTs prints what the reference's printPTs prints, byte for byte, on
+    the golden circuits (the parsed PT lists, controls included)."""
+    for c in golden["circuits"]:
+        d = hq.DGM(np.complex64)
+        d.qubits = c["qubits"]
+        r = po.RefDGM(c["qubits"])
+        for name, m in c["gates"].items():
+            d.gates.addGate(name, [complex(a, b) for a, b in m])
+            r.add_gate(name, [complex(a, b) for a, b in m])
+        d.setFunction(c["steps"], c["iterations"])
+        r.set_function(c["steps"], c["iterations"])
+        capfd.readouterr()
+        d.printPTs()
+        ours = capfd.readouterr().out
+        r.lib.ref_print_pts(r.h)
+        theirs = capfd.readouterr().out
+        assert ours == theirs and ours, c["name"]
