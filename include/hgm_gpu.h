@@ -358,12 +358,19 @@ int hgm_dgm_get_ops(const hgm_dgm *d, hgm_op *out, int64_t cap);
 int hgm_dgm_execute(hgm_dgm *d, int iterations);               /* dgm_core.cpp:243-267 */
 int hgm_dgm_execute_function(hgm_dgm *d, const char *function, int iterations);
 int hgm_dgm_measure(hgm_dgm *d, int qubit, int *outcome);      /* dgm_core.cpp:69-125 */
+/* colapse (dgm_core.cpp:129-146): keep the amplitudes whose qubit equals
+ * value, renormalise them, zero the rest, and print the kept probability. */
+int hgm_dgm_colapse(hgm_dgm *d, int qubit, int value);
 /* Marginal distribution over the given qubits: probs[v] for each value v of
  * the masked index bits, listed in ascending mask-value order (dgm_core.cpp:150-162). */
 int hgm_dgm_measure_many(hgm_dgm *d, const int *qubits, int n, int64_t *values, double *probs, int64_t *n_out);
 int hgm_dgm_sync_host(hgm_dgm *d);                             /* resident mode: download */
 int hgm_dgm_stats(const hgm_dgm *d, hgm_stats *out);
 int hgm_dgm_count_ops(hgm_dgm *d, int iterations, int64_t counts[7]); /* CountOps, dgm_core.cpp:270-299 */
+int hgm_dgm_erase(hgm_dgm *d);                                 /* erase: drop the PT list, dgm_core.cpp:41-52 */
+/* validateTuning (dgm_core.cpp:177-241): clamps the region bits to the qubit
+ * count, then HGM_E_TUNING (with the reference's message) for a bad t_GPU tuple. */
+int hgm_dgm_validate_tuning(hgm_dgm *d);
 /* printPTs (dgm_core.cpp:35-39): every PT's fields to stdout, in PT::print's
  * order and formats (common.cpp:68-74). */
 int hgm_dgm_print_pts(const hgm_dgm *d);

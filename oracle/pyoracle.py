@@ -202,6 +202,7 @@ def _load_ref(so: str = REF_SO):
         lib.ref_revert_bits.argtypes = [C.c_int, C.c_int]
         lib.ref_shor.argtypes = [C.c_long, C.POINTER(C.c_int)]
         lib.ref_print_pts.argtypes = [C.c_void_p]
+        lib.ref_colapse.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.ref_set_gpu_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         lib.ref_general_hadamard.argtypes = [C.c_long, C.c_int, C.c_int]
         lib.ref_grover_run.argtypes = [C.c_long, C.c_long, C.c_int, C.c_int, C.c_ulong]

@@ -87,6 +87,11 @@ int ref_set_function_steps(void *p, const char *const *steps, int n, int iterati
 
 int ref_qubits(void *p) { return (int)static_cast<DGM *>(p)->qubits; }
 
+void ref_colapse(void *p, int qubit, int value) {
+    static_cast<DGM *>(p)->colapse(qubit, value);
+    std::fflush(stdout);
+}
+
 void ref_print_pts(void *p) {
     static_cast<DGM *>(p)->printPTs();
     std::fflush(stdout);

@@ -150,11 +150,14 @@ SIGNATURES = [
     ("hgm_dgm_execute", C.c_int, [_P, C.c_int]),
     ("hgm_dgm_execute_function", C.c_int, [_P, C.c_char_p, C.c_int]),
     ("hgm_dgm_measure", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    ("hgm_dgm_colapse", C.c_int, [_P, C.c_int, C.c_int]),
     ("hgm_dgm_measure_many", C.c_int, [_P, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int64),
                                        C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     ("hgm_dgm_sync_host", C.c_int, [_P]),
     ("hgm_dgm_stats", C.c_int, [_P, C.POINTER(Stats)]),
     ("hgm_dgm_count_ops", C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
+    ("hgm_dgm_erase", C.c_int, [_P]),
+    ("hgm_dgm_validate_tuning", C.c_int, [_P]),
     ("hgm_dgm_print_pts", C.c_int, [_P]),
     ("hgm_seed_rng", None, [C.c_uint64]),
     ("hgm_shor", C.c_int, [_P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(Stats)]),

@@ -207,6 +207,16 @@ int hgm_dgm_measure(hgm_dgm *d, int qubit, int *outcome) {
     return guard([&] { *outcome = visit(d, [&](auto &x) { return x.measure(qubit); }); return HGM_OK; });
 }
 
+int hgm_dgm_colapse(hgm_dgm *d, int qubit, int value) {
+    if (!d) return HGM_E_ARG;
+    return guard([&] {
+        if (qubit < 0 || qubit >= (int)d->core().qubits || (value != 0 && value != 1))
+            throw hgm::Error(HGM_E_ARG, "colapse: qubit outside the state or value not 0/1");
+        visit(d, [&](auto &x) { x.colapse(qubit, value); return 0; });
+        return HGM_OK;
+    });
+}
+
 int hgm_dgm_measure_many(hgm_dgm *d, const int *qubits, int n, int64_t *values, double *probs, int64_t *n_out) {
     if (!d || (!qubits && n) || !n_out) return HGM_E_ARG;
     return guard([&] {
@@ -231,6 +241,17 @@ int hgm_dgm_stats(const hgm_dgm *d, hgm_stats *out) {
     if (!d || !out) return HGM_E_ARG;
     *out = d->core().last_stats;
     return HGM_OK;
+}
+
+int hgm_dgm_erase(hgm_dgm *d) {
+    if (!d) return HGM_E_ARG;
+    d->core().erase();
+    return HGM_OK;
+}
+
+int hgm_dgm_validate_tuning(hgm_dgm *d) {
+    if (!d) return HGM_E_ARG;
+    return guard([&] { d->core().validateTuning(); return HGM_OK; });
 }
 
 int hgm_dgm_count_ops(hgm_dgm *d, int iterations, int64_t counts[7]) {

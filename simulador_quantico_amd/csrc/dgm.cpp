@@ -445,12 +445,21 @@ void BasicDGM<Real>::colapse(int qubit_pos, int value) {
     syncHost();
     const int shift = (int)(qubits - 1 - qubit_pos);
     const size_t n = size_t(1) << qubits;
-    double sum = 0;
+    // The sum is kept in the amplitude type, each |a|^2 formed in double and
+    // added in double, and printed, as src/core/dgm_core.cpp:133-139 does for
+    // its float state; the kept amplitudes are divided by the square root in
+    // that type (:141-145), so the FP32 executor reproduces the reference.
+    Real sum = 0;
     for (size_t i = 0; i < n; ++i)
-        if ((int)((i >> shift) & 1) == value) sum += std::norm(std::complex<double>(state[i]));
-    const double norm = std::sqrt(sum);
+        if ((int)((i >> shift) & 1) == value) {
+            const double re = state[i].real(), im = state[i].imag();
+            sum = (Real)((double)sum + (re * re + im * im));
+        }
+    std::printf("%g\n", (double)(float)sum);
+    std::fflush(stdout);
+    const Real norm = (Real)std::sqrt((double)sum);
     for (size_t i = 0; i < n; ++i)
-        state[i] = ((int)((i >> shift) & 1) == value) ? amp_t(std::complex<double>(state[i]) / norm) : amp_t(0);
+        state[i] = ((int)((i >> shift) & 1) == value) ? amp_t(state[i].real() / norm, state[i].imag() / norm) : amp_t(0);
     dev_valid_ = false;
 }
 

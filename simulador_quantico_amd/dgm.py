@@ -88,6 +88,29 @@ class DGM:
         else:
             object.__setattr__(self, name, value)
 
+    # ---- configuration (include/core/dgm.h:113-116, src/core/dgm_core.cpp:30-32,302-318) ----
+    def setExecType(self, exec_type: int) -> None:
+        self.exec_type = exec_type
+
+    def setCpuStructure(self, cpu_region_bits: int, cpu_coalesced_bits: int) -> None:
+        self.cpu_region_bits = cpu_region_bits
+        self.cpu_coalesced_bits = cpu_coalesced_bits
+
+    def setGpuStructure(self, gpu_coalesced_bits: int, gpu_region_bits: int, repeat_count: int = 1) -> None:
+        """block_size follows from the tile invariant 2*block_size*repeat_count == 2^gpu_region_bits
+        (the reference writes 1 << r/2/R, which C parses as 1 << (r/2/R), src/core/dgm_core.cpp:317)."""
+        self.gpu_coalesced_bits = gpu_coalesced_bits
+        self.gpu_region_bits = gpu_region_bits
+        self.repeat_count = repeat_count
+        self.block_size = (1 << gpu_region_bits) // 2 // repeat_count
+
+    def validateTuning(self) -> None:
+        check(lib.hgm_dgm_validate_tuning(self._h))
+
+    def erase(self) -> None:
+        """Drop the compiled PT list (src/core/dgm_core.cpp:41-52)."""
+        check(lib.hgm_dgm_erase(self._h))
+
     # ---- state -------------------------------------------------------
     def allocateMemory(self) -> None:
         check(lib.hgm_dgm_allocate_memory(self._h))
@@ -157,6 +180,11 @@ class DGM:
         out = C.c_int()
         check(lib.hgm_dgm_measure(self._h, int(qubit), C.byref(out)))
         return out.value
+
+    def colapse(self, qubit: int, value: int) -> None:
+        """Keep the amplitudes whose qubit equals value, renormalised; zero the rest
+        (src/core/dgm_core.cpp:129-146, which also prints the kept probability)."""
+        check(lib.hgm_dgm_colapse(self._h, int(qubit), int(value)))
 
     def printPTs(self) -> None:
         """Every PT's fields to stdout as the reference prints them (src/core/dgm_core.cpp:35-39)."""

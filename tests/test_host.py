@@ -365,3 +365,47 @@ def test_print_pts_matches_reference(capfd, golden):
         r.lib.ref_print_pts(r.h)
         theirs = capfd.readouterr().out
         assert ours == theirs and ours, c["name"]
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built (needs the reference tree)")
+def test_colapse_matches_reference(capfd):
+    """colapse on an FP32 host state: the same amplitudes bit for bit and the
+    same printed probability as the reference's DGM::colapse."""
+    rng = np.random.default_rng(31)
+    n = 9
+    for qubit, value in [(0, 1), (4, 0), (8, 1)]:
+        psi = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(np.complex64)
+        psi /= np.float32(np.linalg.norm(psi))
+        d = hq.DGM(np.complex64)
+        d.qubits = n
+        d.allocateMemory()
+        d.state[:] = psi
+        r = po.RefDGM(n)
+        r.set_state(psi)
+        capfd.readouterr()
+        d.colapse(qubit, value)
+        ours = capfd.readouterr().out
+        r.lib.ref_colapse(r.h, qubit, value)
+        theirs = capfd.readouterr().out
+        assert ours == theirs and ours.strip()
+        assert np.array_equal(d.state.view(np.uint32), r.state().view(np.uint32)), (qubit, value)
+
+
+def test_structure_setters_validate_and_erase():
+    """setExecType / setCpuStructure / setGpuStructure / validateTuning / erase
+    (include/core/dgm.h:113-116, src/core/dgm_core.cpp:30-52,177-241,302-318)."""
+    d = hq.DGM()
+    d.qubits = 14
+    d.setExecType(hq.T_GPU)
+    d.setCpuStructure(12, 9)
+    assert (d.exec_type, d.cpu_region_bits, d.cpu_coalesced_bits) == (hq.T_GPU, 12, 9)
+    d.setGpuStructure(3, 12, 8)
+    assert (d.gpu_coalesced_bits, d.gpu_region_bits, d.repeat_count, d.block_size) == (3, 12, 8, 256)
+    d.validateTuning()
+    d.block_size = 128
+    with pytest.raises(hq.HGMError, match="2\\*block_size\\*repeat_count"):
+        d.validateTuning()
+    d.setFunction("H,H,H,H,H,H,H,H,H,H,H,H,H,H")
+    assert len(d.pts()[0]) == 14
+    d.erase()
+    assert len(d.pts()[0]) == 0
