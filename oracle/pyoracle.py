@@ -203,6 +203,8 @@ def _load_ref(so: str = REF_SO):
         lib.ref_shor.argtypes = [C.c_long, C.POINTER(C.c_int)]
         lib.ref_print_pts.argtypes = [C.c_void_p]
         lib.ref_colapse.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        lib.ref_measure_many.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_long]
+        lib.ref_measure_many.restype = C.c_long
         lib.ref_set_gpu_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         lib.ref_general_hadamard.argtypes = [C.c_long, C.c_int, C.c_int]
         lib.ref_grover_run.argtypes = [C.c_long, C.c_long, C.c_int, C.c_int, C.c_ulong]

@@ -17,6 +17,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -86,6 +87,18 @@ int ref_set_function_steps(void *p, const char *const *steps, int n, int iterati
 }
 
 int ref_qubits(void *p) { return (int)static_cast<DGM *>(p)->qubits; }
+
+// measure(vector<int>) (src/core/dgm_core.cpp:150-162): the map's entries in
+// key order; returns their number (at most cap are written).
+long ref_measure_many(void *p, const int *qubits, int n, long *keys, float *probs, long cap) {
+    std::map<long, float> m = static_cast<DGM *>(p)->measure(std::vector<int>(qubits, qubits + n));
+    long k = 0;
+    for (const auto &kv : m) {
+        if (k < cap) { keys[k] = kv.first; probs[k] = kv.second; }
+        ++k;
+    }
+    return k;
+}
 
 void ref_colapse(void *p, int qubit, int value) {
     static_cast<DGM *>(p)->colapse(qubit, value);

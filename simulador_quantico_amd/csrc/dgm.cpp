@@ -433,9 +433,18 @@ std::map<long, double> BasicDGM<Real>::measure(const std::vector<int> &qubit_pos
     syncHost();
     long mask = 0;
     for (int q : qubit_positions) mask |= 1L << (qubits - 1 - q);
-    std::map<long, double> probs;
+    // sums kept in the amplitude type, each |a|^2 formed and added in double,
+    // as the reference's map<long, float> does (src/core/dgm_core.cpp:158): the
+    // FP32 executor returns the reference's values exactly
+    std::map<long, Real> sums;
     const size_t n = size_t(1) << qubits;
-    for (size_t i = 0; i < n; ++i) probs[(long)i & mask] += std::norm(std::complex<double>(state[i]));
+    for (size_t i = 0; i < n; ++i) {
+        const double re = state[i].real(), im = state[i].imag();
+        Real &acc = sums[(long)i & mask];
+        acc = (Real)((double)acc + (re * re + im * im));
+    }
+    std::map<long, double> probs;
+    for (const auto &kv : sums) probs[kv.first] = (double)kv.second;
     return probs;
 }
 

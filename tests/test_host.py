@@ -409,3 +409,27 @@ def test_structure_setters_validate_and_erase():
     assert len(d.pts()[0]) == 14
     d.erase()
     assert len(d.pts()[0]) == 0
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built (needs the reference tree)")
+def test_measure_many_matches_reference():
+    """measure(list of qubits) on an FP32 host state returns the reference's
+    map<long, float> exactly: keys are the masked index values, values the float sums."""
+    rng = np.random.default_rng(77)
+    n = 10
+    psi = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(np.complex64)
+    psi /= np.float32(np.linalg.norm(psi))
+    d = hq.DGM(np.complex64)
+    d.qubits = n
+    d.allocateMemory()
+    d.state[:] = psi
+    r = po.RefDGM(n)
+    r.set_state(psi)
+    for qs in ([0], [3, 7], [9, 0, 5], list(range(n))):
+        ours = d.measure(qs)
+        q = np.asarray(qs, np.int32)
+        keys = np.zeros(1 << n, np.int64)
+        vals = np.zeros(1 << n, np.float32)
+        k = r.lib.ref_measure_many(r.h, q.ctypes.data, len(qs), keys.ctypes.data, vals.ctypes.data, 1 << n)
+        theirs = {int(keys[i]): float(vals[i]) for i in range(k)}
+        assert ours == theirs, qs
