@@ -70,7 +70,7 @@ def random_host(rng, n, dtype):
 
 # (host qubits, region mask, region id, dtype): gaps in the mask, a mask without
 # bit 0 (one-amplitude portions), the whole state, and regions of several
-# 16 MiB staging chunks in FP32 and FP64
+# staging chunks in FP32 and FP64 (32, 64 and 256 MiB)
 REGION_CASES = [
     (12, 0b010100111111, 0b001001000000, np.complex64),
     (12, 0b101010101010, 0b000000010001, np.complex128),
@@ -78,6 +78,7 @@ REGION_CASES = [
     (25, ((1 << 15) - 1) | (1 << 16) | (1 << 17) | (1 << 19) | (1 << 20) | (1 << 22) | (1 << 23) | (1 << 24),
      (1 << 18) | (1 << 21), np.complex64),
     (24, ((1 << 24) - 1) & ~((1 << 2) | (1 << 13)), 1 << 13, np.complex128),
+    (27, ((1 << 27) - 1) & ~((1 << 20) | (1 << 26)), 1 << 20, np.complex64),
 ]
 
 

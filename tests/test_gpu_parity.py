@@ -556,3 +556,36 @@ def test_c_program_on_the_pod_api(tmp_path):
     assert float(err) < 1e-15
     assert int(idx) == 0x5a5a5 and abs(float(prob) - 1) < 1e-12 and abs(float(norm) - 1) < 1e-12
     assert int(passes) == 4   # two runs of the two-pass plan (20 bits, 12 in a tile), counts added up
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_one_call_entry_on_pageable_and_pinned_host_states(dtype):
+    """hgm_execute on a pageable host state (a numpy array: staged through the
+    library's pinned chunks) and on a pinned one (the DGM's state: straight
+    DMA) gives the same bits; a Hadamard layer on |0..0> at 26 qubits is the
+    uniform state. Prints the copy rates of both paths."""
+    n = 26
+    h = 1 / np.sqrt(2)
+    ops = hq.make_ops(list(range(n)), [0] * n, [0] * n, [np.array([h, h, h, -h])] * n)
+    code = N.HGM_C128 if dtype == np.complex128 else N.HGM_C64
+    tun = N.default_tuning(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=3)
+    pageable = np.zeros(1 << n, dtype)
+    pageable[0] = 1
+    d = dgm_for(n, dtype)                       # pinned host state
+    d.setMemoryValue(0)
+    pinned = d.state
+    out = {}
+    for name, arr in (("pageable", pageable), ("pinned", pinned)):
+        for _ in range(2):                      # the second call is timed: staging and plan set up
+            arr[:] = 0
+            arr[0] = 1
+            st = N.Stats()
+            check = N.lib.hgm_execute(arr.ctypes.data, n, code, ops, len(ops), C.byref(tun), 1, C.byref(st))
+            assert check == 0, N.lib.hgm_last_error()
+        nbytes = arr.nbytes
+        say(f"{name} {np.dtype(dtype).name}: H2D {nbytes / st.h2d_ms / 1e6:.1f} GB/s, "
+            f"D2H {nbytes / st.d2h_ms / 1e6:.1f} GB/s, kernels {st.kernel_ms:.2f} ms")
+        out[name] = arr.copy()
+    assert np.array_equal(out["pageable"], out["pinned"])
+    tol = 1e-6 if dtype == np.complex64 else 1e-15
+    assert np.abs(out["pinned"] - 2.0 ** (-n / 2)).max() < tol
