@@ -162,7 +162,8 @@ int hgm_release_cache(void);
  * is what the reference's t_GPU does per execute() (src/core/kernel.cu:131-291),
  * with the device buffer cached across calls, and the plan too when the same
  * gate list, tuning and iterations ran recently (stats->plan_ms then holds the
- * lookup time). */
+ * lookup time). A pinned host state (hipHostMalloc) is copied by DMA directly;
+ * a pageable one through the library's pinned staging chunks. */
 int hgm_execute(void *host_state, int qubits, int dtype, const hgm_op *ops, int64_t n_ops,
                 const hgm_tuning *tuning, int iterations, hgm_stats *stats);
 
