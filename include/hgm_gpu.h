@@ -47,7 +47,8 @@ enum {
     HGM_E_NOMEM = 4,       /* device or pinned host allocation failed */
     HGM_E_PARSE = 5,       /* circuit text names an unknown gate or is malformed */
     HGM_E_STATE = 6,       /* state handle / qubit count mismatch */
-    HGM_E_UNSUPPORTED = 7  /* feature outside this library (t_CPU, t_HYBRID, multi-GPU) */
+    HGM_E_UNSUPPORTED = 7  /* feature outside this library (the CPU engines behind the executor's
+                              t_CPU / t_PAR_CPU / t_HYBRID, multi-GPU) */
 };
 
 /* amplitude types */

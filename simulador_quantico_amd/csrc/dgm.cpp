@@ -350,7 +350,10 @@ typename BasicDGM<Real>::amp_t *BasicDGM<Real>::execute(int iterations) {
     if (exec_type != t_GPU) {
         if (exec_type < t_CPU || exec_type > t_HYBRID) fail(HGM_E_ARG, "Erro exec type");
         fail(HGM_E_UNSUPPORTED, "exec_type " + std::to_string(exec_type) +
-                                    " is not provided by this library: only t_GPU (2) runs here");
+                                    " is not provided by this library: only t_GPU (2) runs here" +
+                                    (exec_type == t_HYBRID ? " (t_HYBRID's GPU side, ProjectState / "
+                                                             "GpuExecutionWrapper(NULL) / GetState, serves the "
+                                                             "reference's own engine)" : ""));
     }
     if (!state) fail(HGM_E_STATE, "execute() before allocateMemory()");
     if (qubits != state_qubits)
