@@ -157,7 +157,9 @@ public:
 
     void executeFunction(const std::string &function, int iterations = 1);
     void executeFunction(const std::vector<std::string> &steps, int iterations = 1);
-    amp_t *execute(int iterations);     // src/core/dgm_core.cpp:243-267
+    // src/core/dgm_core.cpp:243-267: t_GPU, and t_HYBRID with every region on the GPU (the PT list
+    // once, as HybridExecution runs it); t_CPU / t_PAR_CPU fail with HGM_E_UNSUPPORTED
+    amp_t *execute(int iterations);
 
     void syncHost();                    // resident mode: bring the host copy up to date
     hgm_state *deviceState() { return dev_; }

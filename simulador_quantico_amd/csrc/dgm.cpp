@@ -255,7 +255,7 @@ hgm_tuning DGMCore::tuning() const {
 // multi-GPU peer mode is not part of this library.
 void DGMCore::validateTuning() {
     if (cpu_region_bits > qubits) cpu_region_bits = qubits;
-    if (exec_type != t_GPU) return;
+    if (exec_type != t_GPU && exec_type != t_HYBRID) return;
     if (gpu_count != 1)
         fail(HGM_E_UNSUPPORTED, "Erro de tuning: gpu_count (" + std::to_string(gpu_count) +
                                     ") must be 1 (one GPU per process; the multi-GPU peer mode is not provided)");
@@ -347,14 +347,14 @@ void BasicDGM<Real>::syncHost() {
 template <class Real>
 typename BasicDGM<Real>::amp_t *BasicDGM<Real>::execute(int iterations) {
     validateTuning();
-    if (exec_type != t_GPU) {
-        if (exec_type < t_CPU || exec_type > t_HYBRID) fail(HGM_E_ARG, "Erro exec type");
+    if (exec_type < t_CPU || exec_type > t_HYBRID) fail(HGM_E_ARG, "Erro exec type");
+    if (exec_type == t_CPU || exec_type == t_PAR_CPU)
         fail(HGM_E_UNSUPPORTED, "exec_type " + std::to_string(exec_type) +
-                                    " is not provided by this library: only t_GPU (2) runs here" +
-                                    (exec_type == t_HYBRID ? " (t_HYBRID's GPU side, ProjectState / "
-                                                             "GpuExecutionWrapper(NULL) / GetState, serves the "
-                                                             "reference's own engine)" : ""));
-    }
+                                    " is not provided by this library: only t_GPU (2) and t_HYBRID (3) run here");
+    // t_HYBRID: the GPU takes every region, the reference's own schedule when
+    // thread_count is 1, and the PT list runs once whatever iterations says,
+    // as DGM::execute calls HybridExecution(pts) (src/core/dgm_core.cpp:262-264)
+    if (exec_type == t_HYBRID) iterations = 1;
     if (!state) fail(HGM_E_STATE, "execute() before allocateMemory()");
     if (qubits != state_qubits)
         fail(HGM_E_STATE, "the circuit has " + std::to_string(qubits) + " qubits but the state was allocated for " +

@@ -589,3 +589,34 @@ def test_one_call_entry_on_pageable_and_pinned_host_states(dtype):
     assert np.array_equal(out["pageable"], out["pinned"])
     tol = 1e-6 if dtype == np.complex64 else 1e-15
     assert np.abs(out["pinned"] - 2.0 ** (-n / 2)).max() < tol
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_executor_t_hybrid_runs_every_region_on_the_gpu(dtype):
+    """exec_type t_HYBRID in the executor: every region on the GPU (the
+    reference's schedule at thread_count 1) and the PT list applied once,
+    whatever execute()'s iterations (DGM::execute calls HybridExecution(pts),
+    src/core/dgm_core.cpp:262-264): the same bits as t_GPU with one iteration."""
+    n = 12
+    rng = np.random.default_rng(12)
+    t, cm, cv, mats = _random_gate_list(rng, n, 80)
+    out = {}
+    for et, iters in ((hq.T_GPU, 1), (hq.T_HYBRID, 3)):
+        d = dgm_for(n, dtype)
+        d.exec_type = et
+        for i, m in enumerate(mats):
+            d.gates.addGate(f"g{i}", list(m))
+        steps = []
+        for i, (tb, m_, v_) in enumerate(zip(t, cm, cv)):
+            tok = ["ID"] * n
+            tok[n - 1 - tb] = f"Target1(g{i})" if m_ else f"g{i}"
+            for b in range(n):
+                if (m_ >> b) & 1:
+                    tok[n - 1 - b] = f"Control1({(v_ >> b) & 1})"
+            steps.append(",".join(tok))
+        d.setMemoryValue(5)
+        d.setFunction(steps)
+        out[et] = d.execute(iters).copy()
+    assert np.array_equal(out[hq.T_GPU], out[hq.T_HYBRID])
+    want = oracle_run_gates(n, t, cm, cv, mats, basis=5)
+    assert np.abs(out[hq.T_GPU] - want).max() < (TOL32 if dtype == np.complex64 else TOL64)

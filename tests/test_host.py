@@ -197,12 +197,11 @@ def test_tuning_contract():
     assert tuning_rc(3) == 0                                            # small states run as one tile
 
 
-def test_exec_types_other_than_gpu_are_rejected():
+def test_cpu_exec_types_are_rejected():
     d = hq.DGM()
     d.qubits = 3
     d.allocateMemory()
-    for et, msg in [(hq.T_CPU, "only t_GPU"), (hq.T_PAR_CPU, "only t_GPU"), (hq.T_HYBRID, "only t_GPU"),
-                    (7, "Erro exec type")]:
+    for et, msg in [(hq.T_CPU, "only t_GPU"), (hq.T_PAR_CPU, "only t_GPU"), (7, "Erro exec type")]:
         d.exec_type = et
         d.setFunction("H,H,H")
         with pytest.raises(hq.HGMError, match=msg):
