@@ -486,15 +486,23 @@ bool ProjectState(hgm_cfloat *state, int qubits, int region_size, long region_id
         g_proj = nullptr;
     }
     int rc = hgm_state_create(region_size, HGM_C64, &g_proj);
+    auto t1 = std::chrono::steady_clock::now();
     if (!rc) rc = hgm_state_gather(g_proj, state, qubits, (uint64_t)region_id, mask);
     if (rc) {
         std::printf("error: %d - %s\n", rc, hgm_last_error());
         std::exit(1);
     }
-    if (debug)
-        std::fprintf(stderr, "[PROJSTATE] region_size=%d coalesced_bits=%d elapsed_ms=%.3f\n", region_size,
-                     __builtin_ctzll(~mask),
-                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    if (debug) {   // the reference's fields (src/core/kernel.cu:374-377); the portions move in one pipelined copy
+        const auto t2 = std::chrono::steady_clock::now();
+        const int c = __builtin_ctzll(~mask);
+        std::fprintf(stderr,
+                     "[PROJSTATE] region_size=%d coalesced_bits=%d mem_portions=%ld portion_size=%ld malloc_ms=%.3f "
+                     "memcpy_ms=%.3f elapsed_ms=%.3f\n",
+                     region_size, c, 1L << (region_size - c), 1L << c,
+                     std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                     std::chrono::duration<double, std::milli>(t2 - t1).count(),
+                     std::chrono::duration<double, std::milli>(t2 - t0).count());
+    }
     return true;
 }
 
@@ -506,16 +514,24 @@ bool GetState(hgm_cfloat *state, int qubits, int region_size, long region_id, lo
     if (!g_proj || hgm_state_qubits(g_proj) != region_size)
         ref_die("GetState: no projected region of " + std::to_string(region_size) + " qubits (ProjectState)");
     int rc = hgm_state_scatter(g_proj, state, qubits, (uint64_t)region_id, mask);
+    auto t1 = std::chrono::steady_clock::now();
     hgm_state_destroy(g_proj);
     g_proj = nullptr;
     if (rc) {
         std::printf("error: %d - %s\n", rc, hgm_last_error());
         std::exit(1);
     }
-    if (debug)
-        std::fprintf(stderr, "[GETSTATE]  region_size=%d coalesced_bits=%d elapsed_ms=%.3f\n", region_size,
-                     __builtin_ctzll(~mask),
-                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    if (debug) {   // the reference's fields (src/core/kernel.cu:430-433)
+        const auto t2 = std::chrono::steady_clock::now();
+        const int c = __builtin_ctzll(~mask);
+        std::fprintf(stderr,
+                     "[GETSTATE]  region_size=%d coalesced_bits=%d mem_portions=%ld portion_size=%ld memcpy_ms=%.3f "
+                     "free_ms=%.3f elapsed_ms=%.3f\n",
+                     region_size, c, 1L << (region_size - c), 1L << c,
+                     std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                     std::chrono::duration<double, std::milli>(t2 - t1).count(),
+                     std::chrono::duration<double, std::milli>(t2 - t0).count());
+    }
     return true;
 }
 
