@@ -245,7 +245,7 @@ def _low_control_circuit(rng, n, n_steps, names):
 
 @pytest.mark.skipif(not po.ref_available(po.REF_HGM_SO),
                     reason="oracle/_ref/libhgmref_hgm.so not built (needs the reference tree)")
-@pytest.mark.parametrize("n,threads", [(21, 1), (21, 3), (16, 2)])
+@pytest.mark.parametrize("n,threads", [(21, 1), (21, 3), (16, 2), (23, 8)])
 def test_reference_engine_linked_against_libhgm(n, threads):
     """The reference's DGM with this library in place of its GPU stub: t_GPU and
     t_HYBRID (thread 0 drives the GPU through ProjectState /
