@@ -158,10 +158,15 @@ public:
     void executeFunction(const std::string &function, int iterations = 1);
     void executeFunction(const std::vector<std::string> &steps, int iterations = 1);
     // src/core/dgm_core.cpp:243-267: t_GPU, and t_HYBRID with every region on the GPU (the PT list
-    // once, as HybridExecution runs it); t_CPU / t_PAR_CPU fail with HGM_E_UNSUPPORTED
+    // once, as HybridExecution runs it); t_CPU / t_PAR_CPU fail with HGM_E_UNSUPPORTED. Returns the
+    // updated host state; in resident mode the result stays on the device and nullptr is returned
+    // (syncHost() or hostState() brings the host copy up to date)
     amp_t *execute(int iterations);
 
     void syncHost();                    // resident mode: bring the host copy up to date
+    // syncHost(), then hand out the host copy for reading and writing: in resident mode the
+    // device copy is treated as stale, so host writes reach the next execute()
+    amp_t *hostState();
     hgm_state *deviceState() { return dev_; }
 
 private:

@@ -201,6 +201,10 @@ int hgm_state_apply(hgm_state *s, const hgm_op *ops, int64_t n_ops, const hgm_tu
 int hgm_state_norm2(const hgm_state *s, double *out);
 /* P(qubit == 1) (FP64 accumulation on the device). qubit counts from the MSB. */
 int hgm_state_prob_one(const hgm_state *s, int qubit, double *out);
+/* P(qubit == 0) and P(qubit == 1), each summed directly over its half (FP64
+ * accumulation on the device): the two sums of DGM::measure
+ * (src/core/dgm_core.cpp:79-84). qubit counts from the MSB. */
+int hgm_state_prob_halves(const hgm_state *s, int qubit, double *zero, double *one);
 /* Collapse qubit to `outcome` and move the surviving half to the |0> slot,
  * dividing by sqrt(p) (src/core/dgm_core.cpp:115-122). */
 int hgm_state_collapse_reset(hgm_state *s, int qubit, int outcome, double p);
@@ -350,7 +354,10 @@ int hgm_dgm_get(const hgm_dgm *d, const char *field, int64_t *value);
 int hgm_dgm_allocate_memory(hgm_dgm *d);                       /* dgm_core.cpp:54-56 */
 int hgm_dgm_free_memory(hgm_dgm *d);
 int hgm_dgm_set_memory_value(hgm_dgm *d, int64_t pos);         /* dgm_core.cpp:63-65 */
-void *hgm_dgm_state(hgm_dgm *d);                               /* host amplitudes */
+/* Host amplitudes, brought up to date first; in resident mode the device copy
+ * is then treated as stale, so writes through the pointer reach the next
+ * execute. NULL when no state is allocated or the copy fails. */
+void *hgm_dgm_state(hgm_dgm *d);
 int hgm_dgm_add_gate(hgm_dgm *d, const char *name, const double *m8, int *added); /* gates.cpp:42-60 */
 int hgm_dgm_get_gate(const hgm_dgm *d, const char *name, double *m8, int *found);
 int hgm_dgm_set_function(hgm_dgm *d, const char *function, int iterations, int reset); /* dgm_parser.cpp:29-35 */

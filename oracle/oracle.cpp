@@ -351,23 +351,31 @@ struct Oracle {
     }
 
     // ---------------- readout (dgm_core.cpp:69-162) ----------------
+    // measure(qubit) follows dgm_core.cpp:75-122: P(0) and P(1) are summed in
+    // index order into accumulators of the amplitude type (the reference's
+    // `float zero, one`, :75), each term formed in double (pow(float, 2.0),
+    // :81,83); the double draw is rounded to float before the test
+    // `zero > random_sample` (:98-99); the kept half is divided by sqrt(P)
+    // taken in the amplitude type (:106-109,118). With float amplitudes every
+    // step is the reference's own arithmetic.
     int measure(int qubit) {
         const int64_t size = int64_t(1) << qubits;
         const int shift = qubits - 1 - qubit;
-        // the reference accumulates in float; the oracle accumulates in double
-        double one = 0, zero = 0;
+        R one = 0, zero = 0;
         for (int64_t p = 0; p < size; ++p) {
-            double pr = double(state[p].real()) * state[p].real() + double(state[p].imag()) * state[p].imag();
-            if ((p >> shift) & 1) one += pr; else zero += pr;
+            const double re = state[p].real(), im = state[p].imag();
+            const double pr = re * re + im * im;
+            if ((p >> shift) & 1) one = R(double(one) + pr); else zero = R(double(zero) + pr);
         }
         std::uniform_real_distribution<double> dist(0.0, 1.0);
-        double r = dist(rng);
-        int bit = (zero > r) ? 0 : 1;
-        double norm = std::sqrt(bit ? one : zero);
+        const float r = float(dist(rng));
+        const int bit = (zero > R(r)) ? 0 : 1;
+        const R norm = R(std::sqrt(double(bit ? one : zero)));
         const int64_t low = (int64_t(1) << shift) - 1;
         for (int64_t p = 0; p < size / 2; ++p) {
-            int64_t pos = (p << 1) - (p & low);
-            state[pos] = C(std::complex<double>(state[pos | (int64_t(bit) << shift)]) / norm);
+            const int64_t pos = (p << 1) - (p & low);
+            const C v = state[pos | (int64_t(bit) << shift)];
+            state[pos] = C(v.real() / norm, v.imag() / norm);
             state[pos | (int64_t(1) << shift)] = C(0);
         }
         return bit;
@@ -512,6 +520,39 @@ void orc_seed(void *p, unsigned long seed) {
 
 int orc_measure(void *p, int qubit) {
     return with(static_cast<Handle *>(p), [&](auto &o) { return o.measure(qubit); });
+}
+
+// Fixture search (tests/golden/make_golden.py): the smallest seed >= start,
+// below start + count, whose first uniform_real_distribution<double>(0, 1)
+// draw from a std::mt19937 seeded with it (src/core/common.cpp:10-14, the
+// draw of src/core/dgm_core.cpp:97) lies in [lo, hi); -1 if none. Searched on
+// `threads` OpenMP threads.
+long orc_find_draw_seed(double lo, double hi, long start, long count, int threads) {
+    std::atomic<long> best{-1};
+    const long chunk = 1 << 16;
+    std::atomic<long> next{0};
+    #pragma omp parallel num_threads(threads < 1 ? 1 : threads)
+    {
+        std::mt19937 g;
+        std::uniform_real_distribution<double> dist(0.0, 1.0);
+        for (;;) {
+            const long c0 = next.fetch_add(chunk);
+            const long b = best.load();
+            if (c0 >= count || (b >= 0 && start + c0 > b)) break;
+            const long c1 = std::min(count, c0 + chunk);
+            for (long k = c0; k < c1; ++k) {
+                g.seed((unsigned long)(start + k));
+                dist.reset();
+                const double r = dist(g);
+                if (r >= lo && r < hi) {
+                    long cur = best.load();
+                    while ((cur < 0 || start + k < cur) && !best.compare_exchange_weak(cur, start + k)) {}
+                    break;
+                }
+            }
+        }
+    }
+    return best.load();
 }
 
 }  // extern "C"

@@ -140,7 +140,11 @@ int hgm_dgm_set_memory_value(hgm_dgm *d, int64_t pos) {
 }
 
 void *hgm_dgm_state(hgm_dgm *d) {
-    return d ? visit(d, [](auto &x) { return static_cast<void *>(x.state); }) : nullptr;
+    if (!d) return nullptr;
+    void *p = nullptr;
+    if (guard([&] { p = visit(d, [](auto &x) { return static_cast<void *>(x.hostState()); }); return HGM_OK; }))
+        return nullptr;
+    return p;
 }
 
 int hgm_dgm_add_gate(hgm_dgm *d, const char *name, const double *m8, int *added) {

@@ -338,6 +338,15 @@ void BasicDGM<Real>::ensure_device() {
 }
 
 template <class Real>
+typename BasicDGM<Real>::amp_t *BasicDGM<Real>::hostState() {
+    syncHost();
+    // the caller may write through the pointer: the host copy becomes the
+    // one the next device use uploads
+    if (resident) dev_valid_ = false;
+    return state;
+}
+
+template <class Real>
 void BasicDGM<Real>::syncHost() {
     if (host_valid_ || !dev_ || !dev_valid_) { host_valid_ = true; return; }
     if (int rc = hgm_state_download(dev_, state)) fail(rc, hgm_last_error());
@@ -367,6 +376,7 @@ typename BasicDGM<Real>::amp_t *BasicDGM<Real>::execute(int iterations) {
         if (int rc = hgm_state_apply(dev_, o.data(), (int64_t)o.size(), &t, iterations, &last_stats))
             fail(rc, hgm_last_error());
         host_valid_ = false;
+        return nullptr;   // the result stays on the device: syncHost() or hostState() reads it
     } else {
         if (dev_) { hgm_state_destroy(dev_); dev_ = nullptr; dev_valid_ = false; }
         if (int rc = hgm_execute(state, (int)qubits, dtype(), o.data(), (int64_t)o.size(), &t, iterations, &last_stats))
@@ -391,38 +401,53 @@ void BasicDGM<Real>::executeFunction(const std::vector<std::string> &steps, int 
 // dgm_core.cpp:69-125: probabilities of the two halves, one uniform draw in
 // [0,1) from g_rng, outcome 0 iff P(0) > draw, then the kept half moves to
 // the bit=0 slot divided by sqrt(P) and the bit=1 slot is zeroed (the qubit is
-// reset to |0>). Sums are accumulated in double here.
+// reset to |0>). As in the reference (:75-99), P(0) and P(1) are kept in the
+// amplitude type, each |a|^2 formed in double and added in index order, and
+// the double draw is rounded to float before the test `zero > draw`; sqrt(P)
+// and the division are taken in the amplitude type (:106-109,118). The FP32
+// executor's host path therefore repeats the reference's arithmetic. On a
+// device-resident state both halves are summed on the device in FP64 (each
+// directly, neither as the total minus the other) and then rounded to the
+// amplitude type: the summation order differs from the reference's.
 template <class Real>
 int BasicDGM<Real>::measure(int qubit_pos) {
     if (qubit_pos < 0 || qubit_pos >= qubits) fail(HGM_E_ARG, "measure: qubit out of range");
-    double zero = 0, one = 0;
+    Real zero = 0, one = 0;
     const bool on_device = resident && dev_ && dev_valid_ && !host_valid_;
     const int shift = (int)(qubits - 1 - qubit_pos);
     if (on_device) {
-        double total = 0;
-        if (int rc = hgm_state_prob_one(dev_, qubit_pos, &one)) fail(rc, hgm_last_error());
-        if (int rc = hgm_state_norm2(dev_, &total)) fail(rc, hgm_last_error());
-        zero = total - one;
+        double z = 0, o = 0;
+        if (int rc = hgm_state_prob_halves(dev_, qubit_pos, &z, &o)) fail(rc, hgm_last_error());
+        zero = (Real)z;
+        one = (Real)o;
     } else {
         syncHost();
         const size_t n = size_t(1) << qubits;
         for (size_t i = 0; i < n; ++i) {
-            const double p = std::norm(std::complex<double>(state[i]));
-            if ((i >> shift) & 1) one += p; else zero += p;
+            const double re = state[i].real(), im = state[i].imag();
+            const double p = re * re + im * im;
+            if ((i >> shift) & 1) one = (Real)((double)one + p); else zero = (Real)((double)zero + p);
         }
     }
     std::uniform_real_distribution<double> dist(0.0, 1.0);
-    const double r = dist(g_rng);
-    const int bit = (zero > r) ? 0 : 1;
-    measure_value = bit ? one : zero;
+    const float r = (float)dist(g_rng);
+    const int bit = (zero > (Real)r) ? 0 : 1;
+    const Real kept = bit ? one : zero;
+    measure_value = (double)kept;
+    const Real norm = (Real)std::sqrt((double)kept);
     if (on_device) {
-        if (int rc = hgm_state_collapse_reset(dev_, qubit_pos, bit, measure_value)) fail(rc, hgm_last_error());
+        // the kernel divides in double by sqrt(p). For float amplitudes
+        // p = norm^2 is exact in double, so it divides by `norm` itself, and
+        // a float divided in double and rounded to float is the float
+        // quotient; for double amplitudes sqrt(kept) is `norm`
+        const double p = sizeof(Real) == 4 ? (double)norm * (double)norm : (double)kept;
+        if (int rc = hgm_state_collapse_reset(dev_, qubit_pos, bit, p)) fail(rc, hgm_last_error());
     } else {
-        const double norm = std::sqrt(measure_value);
         const size_t half = size_t(1) << (qubits - 1), low = (size_t(1) << shift) - 1;
         for (size_t i = 0; i < half; ++i) {
             const size_t pos = (i << 1) - (i & low);
-            state[pos] = amp_t(std::complex<double>(state[pos | (size_t(bit) << shift)]) / norm);
+            const amp_t v = state[pos | (size_t(bit) << shift)];
+            state[pos] = amp_t(v.real() / norm, v.imag() / norm);
             state[pos | (size_t(1) << shift)] = amp_t(0);
         }
         dev_valid_ = false;

@@ -828,6 +828,12 @@ int hgm_state_prob_one(const hgm_state *s, int qubit, double *out) {
     return rc;
 }
 
+int hgm_state_prob_halves(const hgm_state *s, int qubit, double *zero, double *one) {
+    if (!s || !zero || !one || qubit < 0 || qubit >= s->qubits) { t_err = "bad qubit"; return HGM_E_ARG; }
+    std::lock_guard<std::mutex> g(ctx().mu);
+    return prob_halves_locked(s, s->qubits - 1 - qubit, zero, one);
+}
+
 int hgm_state_collapse_reset(hgm_state *s, int qubit, int outcome, double p) {
     if (!s || qubit < 0 || qubit >= s->qubits || (outcome != 0 && outcome != 1) || !(p > 0)) {
         t_err = "bad collapse arguments";

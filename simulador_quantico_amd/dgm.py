@@ -123,7 +123,9 @@ class DGM:
 
     @property
     def state(self) -> np.ndarray:
-        """Zero-copy view of the host amplitudes (valid until freeMemory)."""
+        """Zero-copy view of the host amplitudes (valid until freeMemory). Brought up
+        to date first; in resident mode the device copy is then treated as stale,
+        so writes through the view reach the next execute()."""
         p = lib.hgm_dgm_state(self._h)
         if not p:
             raise HGMError(6, "no state allocated")
@@ -154,9 +156,12 @@ class DGM:
         """The compiled PT list as (target_bits, control_masks, control_values, matrices[n,4])."""
         return ops_to_arrays(self.ops())
 
-    def execute(self, iterations: int = 1) -> np.ndarray:
+    def execute(self, iterations: int = 1):
+        """Run the PT list on the GPU (src/core/dgm_core.cpp:243-267) and return the
+        host state; in resident mode the result stays on the device and None is
+        returned (read `state`, which copies it back, or measure on the device)."""
         check(lib.hgm_dgm_execute(self._h, iterations))
-        return self.state
+        return None if self.resident else self.state
 
     def executeFunction(self, function, iterations: int = 1) -> None:
         if isinstance(function, str):
@@ -298,6 +303,12 @@ class State:
         v = C.c_double()
         check(lib.hgm_state_prob_one(self._h, qubit, C.byref(v)))
         return v.value
+
+    def prob_halves(self, qubit: int):
+        """(P(qubit == 0), P(qubit == 1)), each summed on the device in FP64."""
+        z, o = C.c_double(), C.c_double()
+        check(lib.hgm_state_prob_halves(self._h, qubit, C.byref(z), C.byref(o)))
+        return z.value, o.value
 
     def collapse_reset(self, qubit: int, outcome: int, p: float) -> None:
         check(lib.hgm_state_collapse_reset(self._h, qubit, outcome, p))

@@ -38,5 +38,6 @@ def golden():
         "parse": load("parse.json"),
         "circuits": load("circuits.json"),
         "measure": load("measure.json"),
+        "measure_edges": load("measure_edges.json"),
         "states": dict(np.load(os.path.join(GOLDEN, "states.npz"))),
     }

@@ -12,8 +12,12 @@ outputs are data only:
   circuits.json circuit texts + custom FP32 gate matrices of the state fixtures
   states.npz    final FP32 states of those circuits under t_CPU and t_PAR_CPU
   measure.json  measured bits of Grover runs at fixed seeds
+  measure_edges.json  measure(qubit) on states and seeds chosen so that the
+                reference's float arithmetic (float sums of P(0), the draw
+                rounded to float, src/core/dgm_core.cpp:75-99) decides the
+                outcome: the draw lies in a rounding window of P(0)
 
-Usage: python tests/golden/make_golden.py   (needs the reference tree)
+Usage: python tests/golden/make_golden.py [shor | measure_edges]   (needs the reference tree)
 """
 from __future__ import annotations
 
@@ -205,6 +209,79 @@ def measure_fixtures():
     return out
 
 
+def _find_seed(lo: float, hi: float, start: int = 1, count: int = 1 << 30) -> int:
+    """Smallest seed whose first uniform_real_distribution<double> draw from
+    std::mt19937 lies in [lo, hi) (searched by the oracle's helper, which uses
+    the same libstdc++ generator and distribution as the reference)."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+    lib.orc_find_draw_seed.restype = C.c_long
+    lib.orc_find_draw_seed.argtypes = [C.c_double, C.c_double, C.c_long, C.c_long, C.c_int]
+    seed = lib.orc_find_draw_seed(lo, hi, start, count, os.cpu_count() or 1)
+    if seed < 0:
+        raise RuntimeError(f"no seed in [{start}, {start + count}) draws in [{lo}, {hi})")
+    return int(seed)
+
+
+def _float_sum(terms) -> float:
+    """The reference's accumulation: float accumulator, double terms (dgm_core.cpp:79-84)."""
+    acc = np.float32(0)
+    for t in terms:
+        acc = np.float32(np.float64(acc) + t)
+    return float(acc)
+
+
+def measure_edge_fixtures():
+    """measure(0) where the reference's float comparison decides the outcome.
+
+    draw_rounding: P(0) is exact in float (0.75 or 0.5) and the draw lies within
+    half a float ulp below it, so the draw rounds up to P(0) and `zero > draw`
+    is false in float (outcome 1) while true in double (outcome 0).
+    float_sum: a seeded 12-qubit state whose float-accumulated P(0) differs
+    from the double sum, with the draw between the two."""
+    out = []
+    for name, amps, p0 in (("draw_rounding_p075", [0.5, 0.5, 0.5, 0, 0.5, 0, 0, 0], 0.75),
+                           ("draw_rounding_p05", [0.5, 0.5, 0, 0, 0.5, 0.5, 0, 0], 0.5)):
+        a = np.array(amps, np.complex64)
+        half_ulp = float(np.spacing(np.float32(p0 / 2)))   # float spacing just below p0
+        seed = _find_seed(p0 - half_ulp / 2, p0)
+        out.append(_measure_case(name, a, seed, window=[p0 - half_ulp / 2, p0]))
+    rng = np.random.default_rng(4242)
+    n = 12
+    best = None
+    for _ in range(64):
+        v = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(np.complex64)
+        v = (v / np.float32(np.linalg.norm(v.astype(np.complex128)))).astype(np.complex64)
+        terms = np.abs(v.astype(np.complex128)) ** 2
+        zero_half = terms[: 1 << (n - 1)]
+        zf, zd = _float_sum(zero_half), float(np.sum(zero_half))
+        if best is None or abs(zf - zd) > abs(best[1] - best[2]):
+            best = (v, zf, zd)
+    v, zf, zd = best
+    ulp = float(np.spacing(np.float32(zf)))
+    if zf > zd:   # double: zd > r false -> 1; float: zf > float(r) true -> 0 while float(r) < zf
+        lo, hi = zd, zf - ulp
+    else:         # double: zd > r true -> 0; float: zf > float(r) false -> 1 once r >= zf
+        lo, hi = zf, zd
+    seed = _find_seed(lo, hi)
+    out.append(_measure_case("float_sum_12q", v, seed, window=[lo, hi], zero_float=zf, zero_double=zd))
+    return out
+
+
+def _measure_case(name, amps, seed, **extra):
+    n = int(np.log2(len(amps)))
+    r = po.RefDGM(n)
+    r.set_state(amps)
+    po.ref_seed(seed)
+    bit = r.measure(0)
+    after = r.state()
+    case = {"name": name, "qubits": n, "qubit": 0, "seed": seed,
+            "amps": [[float(z.real), float(z.imag)] for z in amps], "bit": int(bit),
+            "after": [[float(z.real), float(z.imag)] for z in after]}
+    case.update(extra)
+    return case
+
+
 def shor_fixtures():
     """Shor's circuit builders, classical helpers and whole attempts (t_CPU) of the reference."""
     out = {"strings": [], "catalog": {}, "frac_approx": [], "mul_inv": [], "revert_bits": [], "attempts": []}
@@ -234,6 +311,11 @@ def shor_fixtures():
 def main():
     if not po.ref_available():
         sys.exit("oracle/_ref/libhgmref.so is missing: run `make -C oracle ref` with the reference tree present")
+    if sys.argv[1:] == ["measure_edges"]:
+        with open(os.path.join(HERE, "measure_edges.json"), "w") as f:
+            json.dump(measure_edge_fixtures(), f, indent=0)
+        print("measure_edges.json written to", HERE)
+        return
     if sys.argv[1:] == ["shor"]:
         with open(os.path.join(HERE, "shor.json"), "w") as f:
             json.dump(shor_fixtures(), f, indent=0)

@@ -246,6 +246,58 @@ def test_measure_matches_reference(golden):
             assert bits == case["bits"], (resident, case)
 
 
+def _edge_amps(pairs, dtype):
+    return np.array([complex(re, im) for re, im in pairs], dtype=dtype)
+
+
+def test_measure_float_comparison(golden):
+    """measure(qubit) on states and seeds written by the reference whose draw falls in a
+    float rounding window of P(0) (src/core/dgm_core.cpp:75-99): the FP32 executor gives
+    the reference's bit and collapsed state bit for bit on the host path; where P(0) is
+    exact in float the device-resident path and the FP64 executor agree too."""
+    say("measure edges")
+    for case in golden["measure_edges"]:
+        n, q = case["qubits"], case["qubit"]
+        want_after = _edge_amps(case["after"], np.complex64)
+        exact_p0 = case["name"].startswith("draw_rounding")
+        runs = [(np.complex64, 0)] + ([(np.complex64, 1), (np.complex128, 0), (np.complex128, 1)] if exact_p0 else [])
+        for dtype, resident in runs:
+            d = dgm_for(n, dtype, resident=resident)
+            d.state[:] = _edge_amps(case["amps"], dtype)
+            if resident:   # X twice on the last qubit (exactly the identity) puts the state on the device
+                x_last = ",".join(["ID"] * (n - 1) + ["X"])
+                d.setFunction([x_last, x_last])
+                assert d.execute(1) is None
+            hq.seed_rng(case["seed"])
+            bit = d.measure(q)
+            assert bit == case["bit"], (case["name"], dtype, resident)
+            got = d.state
+            if dtype == np.complex64:
+                assert np.array_equal(got, want_after), (case["name"], resident)
+            else:
+                assert np.abs(got - want_after).max() < 1e-7, (case["name"], resident)
+
+
+def test_resident_state_view():
+    """Resident mode: execute() keeps the result on the device and returns None; the
+    state view brings it back, and writes through the view reach the next execute()."""
+    n = 6
+    for resident in (0, 1):
+        d = dgm_for(n, resident=resident)
+        d.setMemoryValue(0)
+        d.setFunction(hq.Hadamard(n, 0, n))
+        r = d.execute(1)
+        assert (r is None) == bool(resident)
+        assert np.abs(d.state - 2.0 ** (-n / 2)).max() < 1e-15
+        d.state[:] = 0
+        d.state[5] = 1
+        d.setFunction(",".join(["X"] + ["ID"] * (n - 1)))
+        d.execute(1)
+        want = np.zeros(1 << n)
+        want[5 | (1 << (n - 1))] = 1
+        assert np.abs(d.state - want).max() == 0
+
+
 def test_device_readout():
     n = 14
     rng = np.random.default_rng(9)

@@ -201,6 +201,43 @@ def test_measure_matches_reference(golden):
         assert int("".join(map(str, bits)), 2) == v
 
 
+def _amps(pairs, dtype):
+    return np.array([complex(re, im) for re, im in pairs], dtype=dtype)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_measure_float_comparison(golden, precision):
+    """measure(qubit) keeps the reference's float arithmetic (src/core/dgm_core.cpp:75-99):
+    on states and seeds whose draw falls in a rounding window of P(0) (written by the
+    reference, tests/golden/measure_edges.json) the FP32 oracle returns the reference's
+    bit and collapsed state exactly. The cases discriminate: with P(0) and the draw
+    compared in double, each would come out the other way."""
+    for case in golden["measure_edges"]:
+        n = case["qubits"]
+        lo, hi = case["window"]
+        if case["name"].startswith("draw_rounding"):
+            p0 = sum(re * re + im * im for re, im in case["amps"][: 1 << (n - 1)])
+            assert lo < p0 == hi and case["bit"] == 1      # double semantics: p0 > draw -> 0
+            double_bit = 0
+        else:
+            zf, zd = case["zero_float"], case["zero_double"]
+            double_bit = 0 if zd > hi else 1               # zd > every draw in the window, or none
+            assert lo <= min(zf, zd) + 1e-18 and double_bit != case["bit"]
+        o = po.Oracle(n, precision)
+        o.set_state(_amps(case["amps"], np.complex64 if precision == 32 else np.complex128))
+        o.seed(case["seed"])
+        bit = o.measure(case["qubit"])
+        if precision == 32:
+            assert bit == case["bit"], case["name"]
+            assert np.array_equal(o.state(), _amps(case["after"], np.complex64)), case["name"]
+        elif case["name"].startswith("draw_rounding"):
+            # P(0) is exact in double too: the float-rounded draw decides as in the reference
+            assert bit == case["bit"], case["name"]
+        else:
+            # a double sum of P(0) sits on the other side of the draw
+            assert bit == double_bit, case["name"]
+
+
 def _grover_ref_iteration(n, v):
     """lib_grover.cpp:34-51, written out from the reference's documented form."""
     ctrl = ["Target1(X)"] + [f"Control1({(v >> (n - 1 - q)) & 1})" for q in range(1, n)]
