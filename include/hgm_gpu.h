@@ -331,7 +331,7 @@ int hgm_server_wait_all(hgm_server *s);
 /* Tuning the server uses for states of `qubits` qubits. */
 int hgm_server_tuning(hgm_server *s, int qubits, hgm_tuning *out);
 /* Replace that tuning for requests submitted afterwards (kernel must be
- * HGM_KERNEL_REG with 2*repeat_count in {8, 16, 32}; the rest as
+ * HGM_KERNEL_REG with 2*repeat_count in {2, 4, 8, 16, 32}; the rest as
  * hgm_validate_tuning checks it). */
 int hgm_server_set_tuning(hgm_server *s, int qubits, const hgm_tuning *t);
 /* Milliseconds since the server was created (the clock of hgm_result). */

@@ -726,6 +726,8 @@ template <class T>
 hipError_t batch_reg_any(int A, const BatchItem *items, int n_items, int64_t total_tiles, int B, size_t lds,
                          char *arena, hipStream_t s) {
     switch (A) {
+        case 2: return batch_reg<T, 2>(items, n_items, total_tiles, B, lds, arena, s);
+        case 4: return batch_reg<T, 4>(items, n_items, total_tiles, B, lds, arena, s);
         case 8: return batch_reg<T, 8>(items, n_items, total_tiles, B, lds, arena, s);
         case 16: return batch_reg<T, 16>(items, n_items, total_tiles, B, lds, arena, s);
         case 32: return batch_reg<T, 32>(items, n_items, total_tiles, B, lds, arena, s);

@@ -161,6 +161,9 @@ constexpr int kServingPlanner = HGM_PLAN_GREEDY | HGM_PLAN_FUSE | HGM_PLAN_PIVOT
 // asked for: 16 amplitudes per thread; states up to 13 qubits are one tile;
 // larger ones use tiles of 2^12 amplitudes. The always-local low bits make
 // 128-byte runs: 3 bits of 16-byte FP64 amplitudes, 4 bits of 8-byte FP32 ones.
+// Amplitudes per thread the batched register kernel is instantiated for.
+bool serving_amps_ok(int A) { return A == 2 || A == 4 || A == 8 || A == 16 || A == 32; }
+
 hgm_tuning serving_default_tuning(int n, int dtype) {
     const int c = dtype == HGM_C64 ? 4 : 3;
     hgm_tuning t;
@@ -289,6 +292,10 @@ void planner_loop(hgm_server *s) {
         if (rc == HGM_OK && r->plan.kernel != HGM_KERNEL_REG) {
             rc = HGM_E_TUNING;
             err = "serving needs the register kernel for this tile shape";
+        }
+        if (rc == HGM_OK && !serving_amps_ok(2 * r->plan.shape.R)) {   // fails this request only
+            rc = HGM_E_TUNING;
+            err = "the serving kernel holds 2, 4, 8, 16 or 32 amplitudes per thread";
         }
         if (rc == HGM_OK) {
             for (size_t i = 0; i < r->plan.passes.size(); ++i)
@@ -695,7 +702,7 @@ int hgm_server_set_tuning(hgm_server *s, int qubits, const hgm_tuning *t) {
     std::string err;
     if (int rc = make_shape(qubits, s->cfg.dtype, *t, &shape, &err)) { set_error(err); return rc; }
     const int A = 2 * shape.R;
-    if (A != 8 && A != 16 && A != 32) { set_error("the serving kernel holds 8, 16 or 32 amplitudes per thread"); return HGM_E_TUNING; }
+    if (!serving_amps_ok(A)) { set_error("the serving kernel holds 2, 4, 8, 16 or 32 amplitudes per thread"); return HGM_E_TUNING; }
     if (shape.B > reg_kernel_max_threads(A)) { set_error("block_size above the register kernel's limit for this repeat_count"); return HGM_E_TUNING; }
     std::lock_guard<std::mutex> g(s->mu);   // planner threads copy their entry under this lock
     s->tune[qubits] = *t;

@@ -153,3 +153,20 @@ def test_tuning_override_per_qubit_count():
     for j in jobs:
         assert res[j.index]["status"] == 0
         assert S.fidelity(oracle_run_job(j), states[j.index]) >= 1 - 1e-6
+
+
+def test_tiny_states_beside_others():
+    """1-, 2- and 3-qubit requests (2 and 4 amplitudes per thread) run beside larger ones:
+    every request succeeds, each matches the oracle, and wait_all reports no error."""
+    rng = np.random.default_rng(17)
+    jobs = [S.build_job(i, 0.0, kind, n, rng, 4, 3)
+            for i, (kind, n) in enumerate([("hadamard", 1), ("random", 1), ("hadamard", 2), ("random", 2),
+                                           ("random", 3), ("adder", 3), ("random", 12), ("grover", 15)])]
+    say("tiny states:", [(j.kind, j.qubits) for j in jobs])
+    with S.Server(np.complex128, max_active=16, hbm_budget_gib=1) as srv:
+        states, res = run_collect(srv, jobs, np.complex128, together=True)
+        srv.wait_all()
+    for j in jobs:
+        assert res[j.index]["status"] == 0, (j.kind, j.qubits, res[j.index])
+        ref = oracle_run_job(j)
+        assert np.abs(states[j.index] - ref).max() < 1e-12, (j.kind, j.qubits)
