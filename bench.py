@@ -146,46 +146,36 @@ def build_circuit(w):
     return d, d.ops(), iterations, basis, name
 
 
-def cpu_baseline(max_seconds: float = 20.0):
-    """The reference's t_PAR_CPU (FP32) on a bounded sample: one random layer at the
-    largest qubit count whose predicted run time stays under max_seconds."""
-    from oracle import pyoracle as po
-    import simulador_quantico_amd as hq
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    threads = max(1, min(16, cores))
-
-    def run(n):
-        d = hq.DGM()
-        steps = hq.RandomLayers(n, 1, 1, d)
-        names = sorted({tok for s in steps for tok in s.split(",") if tok.startswith("U3_")})
-        if po.ref_available():
-            r = po.RefDGM(n)
-            kind = "reference"
-        else:
-            r = po.Oracle(n, 32)
-            kind = "port"
-        for nm in names:
-            r.add_gate(nm, d.gates.getMatrix(nm))
-        r.set_function(steps)
-        r.set_basis(0)
-        d.setFunction(steps)
-        gates = len(d.pts()[0])
-        t0 = time.perf_counter()
-        r.execute(po.T_PAR_CPU, threads=threads, coalesced_bits=11, region_bits=14)
-        dt = time.perf_counter() - t0
-        return gates * (1 << n) / dt, dt, kind, gates
-
-    rate, dt, kind, _ = run(24)           # calibration
-    n = 24
-    while n < 30 and (2 * (1 << n) * 45 / rate) < max_seconds * 0.6:
-        n += 1
-    rate, dt, kind, gates = run(n)
-    return {"value": rate, "unit": "amp-updates/s", "cores": threads, "kind": kind,
-            "sample": f"t_PAR_CPU (FP32, cpu_region_bits=14, cpu_coalesced_bits=11) on one random layer "
-                      f"({gates} gates) at {n} qubits, {dt:.1f} s, {threads} threads"}
+def cpu_baseline(budget_s: float = 20.0, runs: int = 3) -> dict:
+    """The reference's t_PAR_CPU (FP32, its own precision) and the oracle's FP64
+    restatement of it on one random layer of the benchmark circuit, timed on every
+    physical core of this host (OMP_PROC_BIND=close, OMP_PLACES=cores; one warm-up,
+    median of `runs`), in a child process so that OpenMP starts with that placement."""
+    import subprocess
+    from oracle import cpu_baseline as cb
+    cpu = cb.host_cpu()
+    env = dict(os.environ, OMP_NUM_THREADS=str(cpu["physical_cores"]), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    res = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--runs", str(runs), "--budget-s",
+                          str(budget_s)], cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+    if res.returncode:
+        raise RuntimeError(f"CPU baseline failed: {res.stderr[-2000:]}")
+    rec = json.loads(res.stdout.strip().splitlines()[-1])
+    main = rec.get("reference_fp32") or rec["port_fp64"]
+    kind = "reference" if "reference_fp32" in rec else "port"
+    label = "the reference's own t_PAR_CPU in FP32" if kind == "reference" else "the oracle's t_PAR_CPU in FP64"
+    out = {"value": main["value"], "unit": "amp-updates/s", "cores": rec["threads"], "kind": kind,
+           "sample": f"{label} (cpu_region_bits=14, cpu_coalesced_bits=11) on {rec['circuit']} "
+                     f"({main['gates']} gates), median of {rec['runs']} runs after one warm-up: "
+                     f"{main['seconds_median']:.2f} s, {rec['threads']} threads "
+                     f"(OMP_PROC_BIND={rec['omp_proc_bind']}, OMP_PLACES={rec['omp_places']})",
+           "cpu_model": rec["cpu"]["model"], "physical_cores": rec["cpu"]["physical_cores"],
+           "logical_cpus": rec["cpu"]["logical_cpus"]}
+    if kind == "reference":
+        port = rec["port_fp64"]
+        out["oracle_fp64"] = {"value": port["value"], "unit": "amp-updates/s", "seconds_median": port["seconds_median"],
+                              "what": "the oracle's FP64 restatement of t_PAR_CPU on the same layer and threads "
+                                      "(the GPU line's precision)"}
+    return out
 
 
 def fidelity_report(names=("hadamard20", "grover22", "draper29", "random30", "random32f")) -> dict:

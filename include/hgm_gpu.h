@@ -84,7 +84,17 @@ enum { HGM_PLAN_GREEDY = 0, /* dependency-aware lookahead, any set of tile-local
         * most the next pass could then admit, instead of by the gates it admits alone. Fewer passes
         * on layered circuits (30-qubit random layers, 2^12 tiles: 11 instead of 12), at about
         * 20 times the planning work. */
-       HGM_PLAN_LOOKAHEAD = 256 };
+       HGM_PLAN_LOOKAHEAD = 256,
+       /* flag: like HGM_PLAN_PIVOT, but each uncontrolled dense gate V (after absorbing the
+        * carried factor) is split as V = diag(x0, x1) * O * diag(1, y) with O real: the phase
+        * diag(1, y) goes in front of O as a diagonal gate kept in O's pass, O is row-scaled
+        * (real: 4 multiply-adds per pair), and diag(x0, x1) with the row factors is carried
+        * forward. The plan-compiled kernels multiply each register slot once by the product of
+        * the consecutive diagonal gates in front of a group of rotations, instead of folding
+        * each phase into its gate. Gates that admit no such split (not a unitary times a scale)
+        * keep the HGM_PLAN_PIVOT form. Subsumes HGM_PLAN_PIVOT. Same unitary; results differ
+        * by rounding only. */
+       HGM_PLAN_ROTATE = 512 };
 
 /* kernel families */
 enum { HGM_KERNEL_AUTO = 0,  /* the register kernel when the tile shape allows it, else LDS */
@@ -258,6 +268,23 @@ int hgm_fuse_gates(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, in
 /* The gate list after row scaling (HGM_PLAN_PIVOT), same conventions (at most 2 * n_ops + qubits + 1
  * gates). */
 int hgm_pivot_rows(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out);
+/* The gate list after the rotation split (HGM_PLAN_ROTATE), same conventions (at most
+ * 2 * n_ops + qubits + 1 gates; the `reserved` field of the phase gates in front of the real
+ * rotations is nonzero). */
+int hgm_rotate_rows(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out);
+
+/* The fast tuning of this library for a state of `qubits` qubits: the
+ * plan-compiled kernels, gate fusion, the rotation split and pass balancing on
+ * the greedy planner, and the tile measured fastest on the 30- and 32-qubit
+ * random-layer circuits (FP64: 256 threads x 8 pairs on 2^12 tiles, 3
+ * coalesced bits; FP32: 512 x 8 on 2^13, 4 coalesced bits). No counterpart in
+ * the reference, whose tuning comes from the command line
+ * (include/cli/cli_common.h:21-25). Callers of the reference's ABI
+ * (GpuExecutionWrapper) and of the executor opt in with the environment
+ * variable HGM_TUNING=auto, or HGM_TUNING=autotune for a tile autotuned once
+ * per qubit count and dtype on the first circuit (hgm_autotune); results are
+ * the same as with the caller's tuning up to rounding. */
+int hgm_auto_tuning(int qubits, int dtype, hgm_tuning *out);
 
 /* Tuning check with the reference's rules plus the gfx950 limits; returns
  * HGM_E_TUNING with a message when the tuple is rejected. */

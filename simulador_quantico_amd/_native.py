@@ -22,6 +22,7 @@ HGM_PLAN_FUSE = 32
 HGM_PLAN_PIVOT = 64
 HGM_PLAN_BALANCE = 128
 HGM_PLAN_LOOKAHEAD = 256
+HGM_PLAN_ROTATE = 512
 HGM_KERNEL_AUTO = 0
 HGM_KERNEL_LDS = 1
 HGM_KERNEL_REG = 2
@@ -134,6 +135,8 @@ SIGNATURES = [
                                  C.POINTER(C.c_int64)]),
     ("hgm_pivot_rows", C.c_int, [C.c_int, C.POINTER(Op), C.c_int64, C.POINTER(Op), C.c_int64,
                                  C.POINTER(C.c_int64)]),
+    ("hgm_rotate_rows", C.c_int, [C.c_int, C.POINTER(Op), C.c_int64, C.POINTER(Op), C.c_int64,
+                                 C.POINTER(C.c_int64)]),
     ("hgm_dgm_create", C.c_int, [C.c_int, C.POINTER(_P)]),
     ("hgm_dgm_destroy", None, [_P]),
     ("hgm_dgm_set", C.c_int, [_P, C.c_char_p, C.c_int64]),
@@ -167,6 +170,7 @@ SIGNATURES = [
     ("hgm_revert_bits", C.c_int, [C.c_int, C.c_int]),
     ("hgm_generate", C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64), C.c_int, C.c_char_p, C.c_char_p, C.c_size_t,
                                C.POINTER(C.c_size_t)]),
+    ("hgm_auto_tuning", C.c_int, [C.c_int, C.c_int, C.POINTER(Tuning)]),
     ("hgm_autotune", C.c_int, [C.c_int, C.c_int, C.POINTER(Op), C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(Tuning),
                                C.POINTER(C.c_double)]),
     ("hgm_server_default_config", None, [C.POINTER(ServerConfig)]),

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -12,6 +13,7 @@
 
 #include "hgm_dgm.hpp"
 #include "hgm_gpu.h"
+#include "planner.hpp"
 #include "runtime.hpp"
 
 static_assert(offsetof(hgm_ref_pt, matrix) == 8, "PT layout (include/core/common.h:57-90)");
@@ -31,6 +33,38 @@ struct hgm_dgm {
         return d64 ? static_cast<const hgm::DGMCore &>(*d64) : static_cast<const hgm::DGMCore &>(*d32);
     }
 };
+
+namespace hgm {
+
+hgm_tuning env_tuning(const hgm_tuning &caller, int qubits, int dtype, const hgm_op *ops, int64_t n_ops,
+                      int iterations) {
+    const char *e = std::getenv("HGM_TUNING");
+    if (!e) return caller;
+    const std::string mode(e);
+    if (mode != "auto" && mode != "autotune") return caller;
+    hgm_tuning t;
+    if (hgm_auto_tuning(qubits, dtype, &t)) return caller;
+    t.ops_block = caller.ops_block;
+    if (mode == "auto") return t;
+    // a tile autotuned once per (qubits, dtype), on the first circuit seen
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, hgm_tuning> tuned;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = tuned.find({qubits, dtype});
+    if (it == tuned.end()) {
+        hgm_tuning best;
+        double ms = 0;
+        if (hgm_autotune(qubits, dtype, ops, n_ops, iterations, t.kernel, t.planner, &best, &ms)) return t;
+        best.kernel = t.kernel;
+        best.planner = t.planner;
+        it = tuned.emplace(std::make_pair(qubits, dtype), best).first;
+    }
+    hgm_tuning r = it->second;
+    r.ops_block = caller.ops_block;
+    return r;
+}
+
+}  // namespace hgm
 
 namespace {
 
@@ -438,6 +472,22 @@ extern "C" {
 
 bool setDevice(int device_id) { return hgm_init(device_id) != HGM_OK; }
 
+int hgm_auto_tuning(int qubits, int dtype, hgm_tuning *out) {
+    if (!out || qubits < 1 || qubits > hgm::kMaxQubits || (dtype != HGM_C128 && dtype != HGM_C64)) {
+        hgm::set_error("bad argument");
+        return HGM_E_ARG;
+    }
+    hgm_default_tuning(out);
+    const bool f32 = dtype == HGM_C64;
+    out->block_size = f32 ? 512 : 256;
+    out->repeat_count = 8;
+    out->gpu_region_bits = f32 ? 13 : 12;
+    out->gpu_coalesced_bits = f32 ? 4 : 3;
+    out->kernel = HGM_KERNEL_JIT;
+    out->planner = HGM_PLAN_GREEDY | HGM_PLAN_FUSE | hgm::kAutoRowFlag | HGM_PLAN_BALANCE;
+    return HGM_OK;
+}
+
 hgm_cfloat *GpuExecutionWrapper(hgm_cfloat *read_memory, hgm_ref_pt **pts, int qubits, int coalesced_bits,
                                 int gpu_region_bits, int gpu_count, int block_size, int repeat_count,
                                 int iterations) {
@@ -456,6 +506,7 @@ hgm_cfloat *GpuExecutionWrapper(hgm_cfloat *read_memory, hgm_ref_pt **pts, int q
     t.gpu_region_bits = gpu_region_bits;
     t.gpu_coalesced_bits = coalesced_bits;
     if (const char *e = std::getenv("HGM_OPS_BLOCK")) t.ops_block = std::atoi(e);
+    t = hgm::env_tuning(t, qubits, HGM_C64, ops.data(), (int64_t)ops.size(), iterations);
     hgm_stats st;
     std::memset(&st, 0, sizeof(st));
     int rc;

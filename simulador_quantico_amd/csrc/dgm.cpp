@@ -369,7 +369,7 @@ typename BasicDGM<Real>::amp_t *BasicDGM<Real>::execute(int iterations) {
         fail(HGM_E_STATE, "the circuit has " + std::to_string(qubits) + " qubits but the state was allocated for " +
                               std::to_string(state_qubits));
     std::vector<hgm_op> o = ops();
-    hgm_tuning t = tuning();
+    const hgm_tuning t = env_tuning(tuning(), (int)qubits, dtype(), o.data(), (int64_t)o.size(), iterations);
     std::memset(&last_stats, 0, sizeof(last_stats));
     if (resident) {
         ensure_device();

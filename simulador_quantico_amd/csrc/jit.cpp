@@ -3,7 +3,9 @@
 
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <chrono>
+#include <complex>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -159,8 +161,9 @@ void emit_pair(const Gen &g, std::string &o, const RegOp &op, int k0, int k1) {
          "; }\n";
 }
 
-void emit_phase(const Gen &g, std::string &o, const RegOp &op, int k) {
-    const double fr = g.round(op.m[0]), fi = g.round(op.m[1]);
+void emit_factor(const Gen &g, std::string &o, double fr, double fi, int k) {
+    fr = g.round(fr);
+    fi = g.round(fi);
     const std::string a = reg(k);
     if (fi == 0.0) {
         if (fr == 1.0) return;
@@ -171,6 +174,68 @@ void emit_phase(const Gen &g, std::string &o, const RegOp &op, int k) {
     o += "        { const " + g.V + " x = " + a + ";\n";
     o += "          " + a + ".x = " + g.lin({{fr, "x.x"}, {-fi, "x.y"}}) + ";\n";
     o += "          " + a + ".y = " + g.lin({{fr, "x.y"}, {fi, "x.x"}}) + "; }\n";
+}
+
+void emit_phase(const Gen &g, std::string &o, const RegOp &op, int k) { emit_factor(g, o, op.m[0], op.m[1], k); }
+
+bool is_phase(const RegOp &o) { return o.kind == R_PHASE || o.kind == R_PHASE_REAL; }
+
+// A phase that tests register bits only: its factor per register slot is a
+// constant of the generated code.
+bool slot_phase(const RegOp &o) { return is_phase(o) && !o.tmask && !o.gmask; }
+
+// Whether two register-program ops commute: phases always do; a phase and a
+// gate on register bit b do when the phase does not depend on b; two gates do
+// when their register bits differ and neither tests the other's bit.
+bool commute(const RegOp &a, const RegOp &b) {
+    const bool pa = is_phase(a), pb = is_phase(b);
+    if (pa && pb) return true;
+    if (pa) return !((a.rmask >> b.rbit) & 1);
+    if (pb) return !((b.rmask >> a.rbit) & 1);
+    return a.rbit != b.rbit && !((a.rmask >> b.rbit) & 1) && !((b.rmask >> a.rbit) & 1);
+}
+
+// Reorders the ops between two remaps into waves: every phase whose
+// predecessors (the earlier ops it does not commute with) are done, then
+// every gate whose predecessors are done, and again. The phases in front of
+// a wave of rotations then form one run, and a run costs one multiply per
+// register slot. The order stays a topological order of the non-commuting
+// pairs, so the product is unchanged.
+void schedule_waves(std::vector<RegOp> &ops) {
+    size_t a = 0;
+    while (a < ops.size()) {
+        size_t b = a;
+        while (b < ops.size() && ops[b].kind != R_REMAP) ++b;
+        const size_t n = b - a;
+        if (n > 2) {
+            std::vector<std::vector<uint32_t>> succ(n);
+            std::vector<uint32_t> preds(n, 0);
+            for (size_t i = 0; i < n; ++i)
+                for (size_t j = i + 1; j < n; ++j)
+                    if (!commute(ops[a + i], ops[a + j])) { succ[i].push_back((uint32_t)j); ++preds[j]; }
+            std::vector<RegOp> out;
+            out.reserve(n);
+            std::vector<char> done(n, 0);
+            size_t emitted = 0;
+            while (emitted < n) {
+                for (int want_phase = 1; want_phase >= 0; --want_phase) {
+                    bool again = true;
+                    while (again) {
+                        again = false;
+                        for (size_t i = 0; i < n; ++i) {
+                            if (done[i] || preds[i] || is_phase(ops[a + i]) != (want_phase == 1)) continue;
+                            done[i] = 1;
+                            ++emitted;
+                            out.push_back(ops[a + i]);
+                            for (uint32_t j : succ[i]) if (--preds[j] == 0 && j < i) again = true;
+                        }
+                    }
+                }
+            }
+            std::copy(out.begin(), out.end(), ops.begin() + a);
+        }
+        a = b + 1;
+    }
 }
 
 std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const std::string &name) {
@@ -211,9 +276,26 @@ std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const st
     for (int k = 0; k < A; ++k)
         o += "        " + g.V + " " + reg(k) + " = s[(base | " + hex64(pm.lreg[k]) + ") + lt];\n";
     int cur = m_first;
-    const RegOp *ops = plan.rops.data() + ps.rop_begin;
+    std::vector<RegOp> ops(plan.rops.begin() + ps.rop_begin, plan.rops.begin() + ps.rop_begin + ps.rop_count);
+    schedule_waves(ops);
+    std::vector<std::complex<double>> factor((size_t)A, std::complex<double>(1, 0));
+    bool pending = false;
+    // one multiply per register slot for the product of a run of slot phases
+    auto flush_factors = [&]() {
+        if (!pending) return;
+        for (int k = 0; k < A; ++k) emit_factor(g, o, factor[k].real(), factor[k].imag(), k);
+        std::fill(factor.begin(), factor.end(), std::complex<double>(1, 0));
+        pending = false;
+    };
     for (int64_t i = 0; i < ps.rop_count; ++i) {
         const RegOp &op = ops[i];
+        if (slot_phase(op)) {
+            for (int k = 0; k < A; ++k)
+                if (((uint32_t)k & op.rmask) == op.rval) factor[k] *= std::complex<double>(op.m[0], op.m[1]);
+            pending = true;
+            continue;
+        }
+        if (!is_phase(op)) flush_factors();   // other phases commute with the pending factors
         if (op.kind == R_REMAP) {
             const int nxt = op.rbit;
             const char *parts[2][2] = {{"", nullptr}, {".x", ".y"}};
@@ -249,6 +331,7 @@ std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const st
         if (cond.empty()) o += body;
         else o += "        if (" + cond + ") {\n" + body + "        }\n";
     }
+    flush_factors();
     for (int k = 0; k < A; ++k)
         o += "        s[(base | " + hex64(pm.sreg[k]) + ") + st] = " + reg(k) + ";\n";
     o += "    }\n}\n";

@@ -58,6 +58,7 @@ void fill_pass_ops(const std::vector<GateInfo> &info, const hgm_op *ops, int64_t
         }
         plan.ops.push_back(d);
         plan.src_index.push_back(gi);
+        plan.pre_phase.push_back(g.kind == K_DIAG && g.need ? 1 : 0);
     }
     pass.op_count = (int64_t)plan.ops.size() - pass.op_begin;
 }
@@ -102,12 +103,28 @@ double gate_cost(const hgm_op &o) {
 // to the one with less, while that lowers the larger of the two: a gate
 // moves to the front of the next pass when it commutes with every gate after
 // it in its own pass, or to the end of the previous pass when it commutes
-// with every gate before it, and only where its target is tile-local. The
-// pass count and the set of gates are unchanged.
+// with every gate before it, and only where its target is tile-local; a
+// phase that rides with the rotation after it (HGM_PLAN_ROTATE) moves with
+// that rotation. The pass count and the set of gates are unchanged.
 void balance_passes(const std::vector<GateInfo> &info, const hgm_op *ops, int64_t n_ops, int64_t cap,
                     const std::vector<uint64_t> &local, std::vector<std::vector<int64_t>> &lists) {
     const size_t P = lists.size();
     if (P < 2) return;
+    // a phase tagged to ride with the rotation after it moves with that rotation
+    auto tagged = [&](int64_t k) { return ops[k % n_ops].reserved == kPrePhaseTag && info[k % n_ops].need; };
+    auto unit_begin = [&](const std::vector<int64_t> &l, int64_t j) {
+        return (j > 0 && tagged(l[j - 1]) && ops[l[j - 1] % n_ops].target_bit == ops[l[j] % n_ops].target_bit) ? j - 1
+                                                                                                                    : j;
+    };
+    auto unit_end = [&](const std::vector<int64_t> &l, int64_t j) {   // one past the unit starting at j
+        return (tagged(l[j]) && j + 1 < (int64_t)l.size() &&
+                ops[l[j + 1] % n_ops].target_bit == ops[l[j] % n_ops].target_bit) ? j + 2 : j + 1;
+    };
+    auto cost_of = [&](const std::vector<int64_t> &l, int64_t a, int64_t b) {
+        double c = 0;
+        for (int64_t k = a; k < b; ++k) c += gate_cost(ops[l[k] % n_ops]);
+        return c;
+    };
     std::vector<double> cost(P, 0.0);
     for (size_t i = 0; i < P; ++i)
         for (int64_t k : lists[i]) cost[i] += gate_cost(ops[k % n_ops]);
@@ -118,22 +135,30 @@ void balance_passes(const std::vector<GateInfo> &info, const hgm_op *ops, int64_
             while (cost[i] > cost[i + 1] && (int64_t)lists[i + 1].size() < cap) {
                 const double gap = cost[i] - cost[i + 1];
                 uint64_t bw = 0, br = 0;
-                int64_t pick = -1;
+                int64_t pa = -1, pb = -1;
                 for (int64_t j = (int64_t)lists[i].size() - 1; j >= 0; --j) {
-                    const GateInfo &g = info[lists[i][j] % n_ops];
-                    const double c = gate_cost(ops[lists[i][j] % n_ops]);
-                    if (!(g.writes & (bw | br)) && !(g.reads & bw) && !(g.need & ~local[i + 1]) && c > 0 && c < gap) {
-                        pick = j;
+                    const int64_t a = unit_begin(lists[i], j);
+                    uint64_t uw = 0, ur = 0, need = 0;
+                    for (int64_t k = a; k <= j; ++k) {
+                        const GateInfo &g = info[lists[i][k] % n_ops];
+                        uw |= g.writes;
+                        ur |= g.reads;
+                        need |= g.need;
+                    }
+                    const double c = cost_of(lists[i], a, j + 1);
+                    if (!(uw & (bw | br)) && !(ur & bw) && !(need & ~local[i + 1]) && c > 0 && c < gap) {
+                        pa = a;
+                        pb = j + 1;
                         break;
                     }
-                    bw |= g.writes;
-                    br |= g.reads;
+                    bw |= uw;
+                    br |= ur;
+                    j = a;
                 }
-                if (pick < 0) break;
-                const int64_t k = lists[i][pick];
-                const double c = gate_cost(ops[k % n_ops]);
-                lists[i].erase(lists[i].begin() + pick);
-                lists[i + 1].insert(lists[i + 1].begin(), k);
+                if (pa < 0) break;
+                const double c = cost_of(lists[i], pa, pb);
+                lists[i + 1].insert(lists[i + 1].begin(), lists[i].begin() + pa, lists[i].begin() + pb);
+                lists[i].erase(lists[i].begin() + pa, lists[i].begin() + pb);
                 cost[i] -= c;
                 cost[i + 1] += c;
                 moved = true;
@@ -142,22 +167,30 @@ void balance_passes(const std::vector<GateInfo> &info, const hgm_op *ops, int64_
             while (cost[i + 1] > cost[i] && (int64_t)lists[i].size() < cap) {
                 const double gap = cost[i + 1] - cost[i];
                 uint64_t bw = 0, br = 0;
-                int64_t pick = -1;
-                for (int64_t j = 0; j < (int64_t)lists[i + 1].size(); ++j) {
-                    const GateInfo &g = info[lists[i + 1][j] % n_ops];
-                    const double c = gate_cost(ops[lists[i + 1][j] % n_ops]);
-                    if (!(g.writes & (bw | br)) && !(g.reads & bw) && !(g.need & ~local[i]) && c > 0 && c < gap) {
-                        pick = j;
+                int64_t pa = -1, pb = -1;
+                for (int64_t j = 0; j < (int64_t)lists[i + 1].size();) {
+                    const int64_t e = unit_end(lists[i + 1], j);
+                    uint64_t uw = 0, ur = 0, need = 0;
+                    for (int64_t k = j; k < e; ++k) {
+                        const GateInfo &g = info[lists[i + 1][k] % n_ops];
+                        uw |= g.writes;
+                        ur |= g.reads;
+                        need |= g.need;
+                    }
+                    const double c = cost_of(lists[i + 1], j, e);
+                    if (!(uw & (bw | br)) && !(ur & bw) && !(need & ~local[i]) && c > 0 && c < gap) {
+                        pa = j;
+                        pb = e;
                         break;
                     }
-                    bw |= g.writes;
-                    br |= g.reads;
+                    bw |= uw;
+                    br |= ur;
+                    j = e;
                 }
-                if (pick < 0) break;
-                const int64_t k = lists[i + 1][pick];
-                const double c = gate_cost(ops[k % n_ops]);
-                lists[i + 1].erase(lists[i + 1].begin() + pick);
-                lists[i].push_back(k);
+                if (pa < 0) break;
+                const double c = cost_of(lists[i + 1], pa, pb);
+                lists[i].insert(lists[i].end(), lists[i + 1].begin() + pa, lists[i + 1].begin() + pb);
+                lists[i + 1].erase(lists[i + 1].begin() + pa, lists[i + 1].begin() + pb);
                 cost[i + 1] -= c;
                 cost[i] += c;
                 moved = true;
@@ -205,8 +238,8 @@ int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::
     if (B > 1024) { *err = "Erro de tuning: block_size must be <= 1024 on gfx950"; return HGM_E_TUNING; }
     if (!is_pow2(B) || !is_pow2(R)) { *err = "Erro de tuning: block_size and repeat_count must be powers of two"; return HGM_E_TUNING; }
     if (t.ops_block < 1) { *err = "Erro de tuning: ops_block must be >= 1"; return HGM_E_TUNING; }
-    const int base_planner =
-        t.planner & ~(HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE | HGM_PLAN_PIVOT | HGM_PLAN_BALANCE | HGM_PLAN_LOOKAHEAD);
+    const int base_planner = t.planner & ~(HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE | HGM_PLAN_PIVOT | HGM_PLAN_BALANCE |
+                                           HGM_PLAN_LOOKAHEAD | HGM_PLAN_ROTATE);
     if (base_planner != HGM_PLAN_GREEDY && base_planner != HGM_PLAN_WINDOW) {
         *err = "unknown planner";
         return HGM_E_TUNING;
@@ -425,6 +458,150 @@ void pivot_rows(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op
     if (g != C(1, 0)) diag_op(0, g, g);
 }
 
+// Phases x0, x1, y of modulus 1 with v[i][j] = x_i * o[i][j] * y_j (y_0 = 1)
+// and every o[i][j] real, for the nonzero entries of v; false when they do
+// not exist up to rounding (more than 16 units in the last place of the
+// entry), as for a matrix that is no unitary times a scale. The nonzero
+// entries link rows to columns; fixing column 0's phase at 1 and walking the
+// links determines the others, and the remaining link (all four entries
+// nonzero) is the check.
+bool real_middle(const std::complex<double> v[2][2], std::complex<double> x[2], std::complex<double> *y,
+                 double o[2][2]) {
+    using C = std::complex<double>;
+    auto unit = [](C z) { return z / std::abs(z); };
+    C row[2], col[2];
+    bool have_r[2] = {false, false}, have_c[2] = {true, false};
+    col[0] = C(1, 0);
+    for (int pass = 0; pass < 3; ++pass)
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) {
+                if (v[i][j] == C(0, 0)) continue;
+                if (have_c[j] && !have_r[i]) { row[i] = unit(v[i][j]) / col[j]; have_r[i] = true; }
+                else if (have_r[i] && !have_c[j]) { col[j] = unit(v[i][j]) / row[i]; have_c[j] = true; }
+            }
+    for (int i = 0; i < 2; ++i) if (!have_r[i]) row[i] = C(1, 0);
+    if (!have_c[1]) col[1] = C(1, 0);
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            const C w = v[i][j] / (row[i] * col[j]);
+            if (std::abs(w.imag()) > 16 * 0x1p-52 * std::abs(w)) return false;
+            o[i][j] = w.real();
+        }
+    x[0] = row[0];
+    x[1] = row[1];
+    *y = col[1];
+    return true;
+}
+
+void rotate_rows(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op> *out) {
+    using C = std::complex<double>;
+    // Still to be applied after the emitted list: g * prod_b diag(1, r[b]) on bit b (as in pivot_rows).
+    std::vector<C> r((size_t)qubits, C(1, 0));
+    C g(1, 0);
+    auto get = [](const hgm_op &o, int k) { return C(o.m[2 * k], o.m[2 * k + 1]); };
+    auto set = [](hgm_op &o, int k, C v) { o.m[2 * k] = v.real(); o.m[2 * k + 1] = v.imag(); };
+    auto diag_op = [&](int b, C a, C d, int32_t tag) {
+        hgm_op o{};
+        o.target_bit = b;
+        o.reserved = tag;
+        set(o, 0, a);
+        set(o, 3, d);
+        out->push_back(o);
+    };
+    auto flush = [&](int b) {
+        if (r[b] == C(1, 0)) return;
+        diag_op(b, C(1, 0), r[b], 0);
+        r[b] = C(1, 0);
+    };
+    auto rescale = [&]() {
+        const double s = std::abs(g);
+        if (s >= 0x1p-24 && s <= 0x1p24) return;
+        diag_op(0, C(s, 0), C(s, 0), 0);
+        g /= s;
+    };
+    out->clear();
+    out->reserve(2 * (size_t)n_ops + (size_t)qubits + 1);
+    for (int64_t i = 0; i < n_ops; ++i) {
+        hgm_op o = ops[i];
+        o.reserved = 0;
+        const int b = o.target_bit;
+        const int kind = classify(o.m);
+        const C u00 = get(o, 0), u11 = get(o, 3);
+        if (kind == K_DIAG) {
+            if (o.control_mask || u00 == C(0, 0)) {   // commutes with every carried factor
+                out->push_back(o);
+                continue;
+            }
+            g *= u00;
+            r[b] *= u11 / u00;
+            rescale();
+            continue;
+        }
+        if (o.control_mask) {            // the factor on b must act on both control halves first
+            flush(b);
+            out->push_back(o);
+            continue;
+        }
+        // V = U * diag(1, r) = diag(x0, x1) * O * diag(1, y), O real
+        const C v[2][2] = {{u00, get(o, 1) * r[b]}, {get(o, 2), u11 * r[b]}};
+        C x[2], y;
+        double om[2][2];
+        const bool zero_row = (v[0][0] == C(0, 0) && v[0][1] == C(0, 0)) || (v[1][0] == C(0, 0) && v[1][1] == C(0, 0));
+        if (kind == K_ANTI || zero_row || !real_middle(v, x, &y, om)) {
+            // the row-scaled complex form of pivot_rows
+            int p[2];
+            C d[2];
+            for (int row = 0; row < 2; ++row) {
+                p[row] = std::abs(v[row][1]) > std::abs(v[row][0]) ? 1 : 0;
+                d[row] = v[row][p[row]];
+            }
+            if (d[0] == C(0, 0) || d[1] == C(0, 0)) {
+                set(o, 0, v[0][0]);
+                set(o, 1, v[0][1]);
+                set(o, 2, v[1][0]);
+                set(o, 3, v[1][1]);
+                r[b] = C(1, 0);
+                out->push_back(o);
+                continue;
+            }
+            for (int row = 0; row < 2; ++row) {
+                set(o, 2 * row + p[row], C(1, 0));
+                set(o, 2 * row + 1 - p[row], v[row][1 - p[row]] / d[row]);
+            }
+            out->push_back(o);
+            g *= d[0];
+            r[b] = d[1] / d[0];
+            rescale();
+            continue;
+        }
+        // the phase y rides in front of the real gate; each row of O is
+        // scaled by its larger entry (exactly 1 after scaling), and the row
+        // factors join x as the factor carried forward
+        if (y != C(1, 0)) diag_op(b, C(1, 0), y, kPrePhaseTag);
+        double d[2];
+        for (int row = 0; row < 2; ++row) {
+            const int pc = std::abs(om[row][1]) > std::abs(om[row][0]) ? 1 : 0;
+            d[row] = om[row][pc];
+            o.m[4 * row + 2 * pc] = 1.0;
+            o.m[4 * row + 2 * pc + 1] = 0.0;
+            o.m[4 * row + 2 * (1 - pc)] = om[row][1 - pc] / d[row];
+            o.m[4 * row + 2 * (1 - pc) + 1] = 0.0;
+        }
+        out->push_back(o);
+        const C l0 = x[0] * d[0], l1 = x[1] * d[1];
+        g *= l0;
+        r[b] = l1 / l0;
+        rescale();
+    }
+    for (int b = 0; b < qubits; ++b) {
+        if (r[b] == C(1, 0)) continue;
+        diag_op(b, g, g * r[b], 0);
+        g = C(1, 0);
+        r[b] = C(1, 0);
+    }
+    if (g != C(1, 0)) diag_op(0, g, g, 0);
+}
+
 int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, const hgm_tuning &t,
                Plan *out, std::string *err) {
     auto t0 = std::chrono::steady_clock::now();
@@ -434,7 +611,8 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
     int st = make_shape(qubits, dtype, t, &plan.shape, err);
     if (st) return st;
     std::vector<hgm_op> expanded, fused, folded;
-    if ((t.planner & (HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE | HGM_PLAN_PIVOT)) && n_ops > 0 && ops) {
+    const bool rotate = (t.planner & HGM_PLAN_ROTATE) != 0;
+    if ((t.planner & (HGM_PLAN_FOLD_PHASES | HGM_PLAN_FUSE | HGM_PLAN_PIVOT | HGM_PLAN_ROTATE)) && n_ops > 0 && ops) {
         for (int64_t i = 0; i < n_ops; ++i)
             if (ops[i].target_bit < 0 || ops[i].target_bit >= qubits || (ops[i].control_mask >> ops[i].target_bit) & 1) {
                 *err = "gate " + std::to_string(i) + ": target bit " + std::to_string(ops[i].target_bit) +
@@ -453,7 +631,11 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
             ops = fused.data();
             n_ops = (int64_t)fused.size();
         }
-        if (t.planner & HGM_PLAN_PIVOT) {   // subsumes phase folding
+        if (rotate) {                       // subsumes row scaling
+            rotate_rows(qubits, ops, n_ops, &folded);
+            ops = folded.data();
+            n_ops = (int64_t)folded.size();
+        } else if (t.planner & HGM_PLAN_PIVOT) {   // subsumes phase folding
             pivot_rows(qubits, ops, n_ops, &folded);
             ops = folded.data();
             n_ops = (int64_t)folded.size();
@@ -490,7 +672,14 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
         GateInfo g;
         g.kind = classify(o.m);
         g.target = o.target_bit;
-        if (g.kind == K_DIAG) { g.need = 0; g.writes = 0; g.reads = tb | o.control_mask; }
+        if (g.kind == K_DIAG) {
+            // a phase in front of a rotation stays with it: its target must be
+            // local, and it is ordered like the rotation against gates on its qubit
+            const bool pre = rotate && o.reserved == kPrePhaseTag;
+            g.need = pre ? tb : 0;
+            g.writes = pre ? tb : 0;
+            g.reads = tb | o.control_mask;
+        }
         else { g.need = tb; g.writes = tb; g.reads = o.control_mask; }
         info[i] = g;
     }
@@ -768,7 +957,8 @@ void build_register_program(Plan &plan) {
                 need[i] = wr[i] = 1u << o.tbit;
                 rd[i] = o.lmask;
             } else if (o.kind == K_DIAG) {
-                need[i] = wr[i] = 0;
+                // a phase in front of a rotation goes to the rotation's register phase
+                need[i] = wr[i] = plan.pre_phase[ps.op_begin + i] ? (1u << o.tbit) : 0;
                 rd[i] = (1u << o.tbit) | o.lmask;
             } else {
                 need[i] = wr[i] = 0;

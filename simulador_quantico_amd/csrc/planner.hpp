@@ -139,6 +139,7 @@ struct Plan {
     std::vector<Pass> passes;
     std::vector<DevOp> ops;
     std::vector<int64_t> src_index;   // index of each DevOp in the iteration-expanded gate list
+    std::vector<uint8_t> pre_phase;   // per DevOp: a kPrePhaseTag gate (kept with the next rotation)
     int64_t gates = 0;
     double plan_ms = 0;
     // register kernel program (kernel == HGM_KERNEL_REG)
@@ -180,6 +181,19 @@ void fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_o
 // last carried diagonal. A dense update then costs 8 real multiply-adds per
 // amplitude pair (4 when the scaled matrix is real) instead of 16.
 void pivot_rows(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op> *out);
+
+// `reserved` tag of the phase gates rotate_rows puts in front of its real
+// rotations: the planners keep such a gate in the pass and register phase of
+// the rotation it precedes (its target must be local there).
+constexpr int32_t kPrePhaseTag = 1;
+
+// Equivalent gate list in which every uncontrolled dense gate V (after the
+// carried factor is absorbed, as in pivot_rows) is split as
+// V = diag(x0, x1) * O * diag(1, y) with O real: diag(1, y) is emitted in
+// front of O, tagged kPrePhaseTag; O is emitted row-scaled (real, an exact 1
+// per row); diag(x0, x1) times the row factors is carried forward. Gates
+// without such a split keep pivot_rows' form.
+void rotate_rows(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op> *out);
 
 // Equivalent gate list in which every run of uncontrolled gates on one qubit
 // (with nothing in between that touches the qubit, except diagonal gates

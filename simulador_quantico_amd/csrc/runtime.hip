@@ -910,7 +910,7 @@ void hgm_plan_destroy(hgm_plan *p) {
 }
 
 namespace {
-enum class Rewrite { fold, fuse, pivot };
+enum class Rewrite { fold, fuse, pivot, rotate };
 
 int rewrite_ops(Rewrite how, int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out) {
     if (!n_out || n_ops < 0 || (n_ops > 0 && !ops) || qubits < 1 || qubits > kMaxQubits) {
@@ -925,6 +925,7 @@ int rewrite_ops(Rewrite how, int qubits, const hgm_op *ops, int64_t n_ops, hgm_o
     std::vector<hgm_op> v;
     if (how == Rewrite::fuse) fuse_gates(qubits, ops, n_ops, &v);
     else if (how == Rewrite::pivot) pivot_rows(qubits, ops, n_ops, &v);
+    else if (how == Rewrite::rotate) rotate_rows(qubits, ops, n_ops, &v);
     else fold_phases(qubits, ops, n_ops, &v);
     *n_out = (int64_t)v.size();
     if (out) {
@@ -945,6 +946,10 @@ int hgm_fuse_gates(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, in
 
 int hgm_pivot_rows(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out) {
     return rewrite_ops(Rewrite::pivot, qubits, ops, n_ops, out, cap, n_out);
+}
+
+int hgm_rotate_rows(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out) {
+    return rewrite_ops(Rewrite::rotate, qubits, ops, n_ops, out, cap, n_out);
 }
 
 int64_t hgm_plan_num_passes(const hgm_plan *p) { return p ? (int64_t)p->plan.passes.size() : -1; }

@@ -41,3 +41,10 @@ def golden():
         "measure_edges": load("measure_edges.json"),
         "states": dict(np.load(os.path.join(GOLDEN, "states.npz"))),
     }
+
+
+@pytest.fixture(scope="session")
+def golden_grover23():
+    import json
+    with open(os.path.join(GOLDEN, "grover23.json")) as f:
+        return json.load(f)

@@ -12,12 +12,15 @@ outputs are data only:
   circuits.json circuit texts + custom FP32 gate matrices of the state fixtures
   states.npz    final FP32 states of those circuits under t_CPU and t_PAR_CPU
   measure.json  measured bits of Grover runs at fixed seeds
+  grover23.json the reference's Grover() (src/algorithms/lib_grover.cpp:15-66) at
+                qubits = 23 (22-bit register + ancilla, 1608 iterations) on
+                its t_PAR_CPU: the bits it measures at a fixed seed
   measure_edges.json  measure(qubit) on states and seeds chosen so that the
                 reference's float arithmetic (float sums of P(0), the draw
                 rounded to float, src/core/dgm_core.cpp:75-99) decides the
                 outcome: the draw lies in a rounding window of P(0)
 
-Usage: python tests/golden/make_golden.py [shor | measure_edges]   (needs the reference tree)
+Usage: python tests/golden/make_golden.py [shor | measure_edges | grover23]   (needs the reference tree)
 """
 from __future__ import annotations
 
@@ -282,6 +285,28 @@ def _measure_case(name, amps, seed, **extra):
     return case
 
 
+def grover23_fixture():
+    """Grover's reference form at 23 qubits, built and measured as Grover() does it
+    (src/algorithms/lib_grover.cpp:30-66): |1,0..0>, H on all, 1608 iterations of
+    oracle + diffusion, then qubits 1..22 measured in order at a fixed seed."""
+    n = 23
+    v = int(np.random.default_rng(23).integers(0, 1 << (n - 1)))
+    it = int(np.pi / 4.0 * np.sqrt(1 << (n - 1)))
+    r = po.RefDGM(n)
+    r.set_function([",".join(["H"] * n)])
+    r.set_function(po.ref_grover_iteration(n, v), it, reset=False)
+    r.set_basis(1 << (n - 1))
+    r.execute(T_PAR_CPU, threads=os.cpu_count() or 1, coalesced_bits=11, region_bits=14)
+    psi = r.state()
+    p = np.abs(psi.astype(np.complex128)) ** 2
+    reg_marginal = p.reshape(2, -1).sum(axis=0)     # qubit 0 (the ancilla) is the top index bit
+    seed = 2323
+    po.ref_seed(seed)
+    bits = [r.measure(q) for q in range(1, n)]
+    return {"qubits": n, "search_value": v, "iterations": it, "seed": seed, "bits": bits,
+            "register_argmax": int(np.argmax(reg_marginal)), "p_register_max": float(reg_marginal.max())}
+
+
 def shor_fixtures():
     """Shor's circuit builders, classical helpers and whole attempts (t_CPU) of the reference."""
     out = {"strings": [], "catalog": {}, "frac_approx": [], "mul_inv": [], "revert_bits": [], "attempts": []}
@@ -311,6 +336,11 @@ def shor_fixtures():
 def main():
     if not po.ref_available():
         sys.exit("oracle/_ref/libhgmref.so is missing: run `make -C oracle ref` with the reference tree present")
+    if sys.argv[1:] == ["grover23"]:
+        with open(os.path.join(HERE, "grover23.json"), "w") as f:
+            json.dump(grover23_fixture(), f, indent=0)
+        print("grover23.json written to", HERE)
+        return
     if sys.argv[1:] == ["measure_edges"]:
         with open(os.path.join(HERE, "measure_edges.json"), "w") as f:
             json.dump(measure_edge_fixtures(), f, indent=0)

@@ -249,3 +249,124 @@ def test_pivot_flag_plans_with_fusion():
     bad = hq.make_ops([n], [0], [0], [np.eye(2)])
     with pytest.raises(N.HGMError):
         pivot(n, bad)
+
+
+def rotate(n, ops):
+    cnt = C.c_int64()
+    N.check(N.lib.hgm_rotate_rows(n, ops, len(ops), None, 0, C.byref(cnt)))
+    out = (N.Op * max(1, cnt.value))()
+    N.check(N.lib.hgm_rotate_rows(n, ops, len(ops), out, cnt.value, C.byref(cnt)))
+    return out if cnt.value else (N.Op * 0)()
+
+
+def _mat(o):
+    return np.array(o.m[0::2]) + 1j * np.array(o.m[1::2])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_rotated_list_is_the_same_unitary(seed):
+    """The rotation split (HGM_PLAN_ROTATE) keeps the unitary, global phase included.
+    Every uncontrolled dense gate it emits for a unitary is real with an exact 1 in
+    each row; every phase gate tagged to ride with a rotation is diag(1, y) and comes
+    right before a real rotation on the same qubit."""
+    n = 7
+    rng = np.random.default_rng(300 + seed)
+    t, cm, cv, mats = _random_gate_list(rng, n, 120)
+    for k in range(0, len(cm), 2):
+        cm[k] = cv[k] = 0
+    ops = hq.make_ops(t, cm, cv, mats)
+    f = rotate(n, ops)
+    assert len(f) <= 2 * len(ops) + n + 1
+    real_rot = tagged = 0
+    for i in range(len(f)):
+        m = _mat(f[i])
+        if f[i].reserved:
+            tagged += 1
+            assert f[i].control_mask == 0 and m[0] == 1 and m[1] == 0 and m[2] == 0 and abs(abs(m[3]) - 1) < 1e-15
+            nxt = _mat(f[i + 1])
+            assert f[i + 1].target_bit == f[i].target_bit and f[i + 1].control_mask == 0
+            assert np.all(nxt.imag == 0)
+            continue
+        if f[i].control_mask or (m[1] == 0 and m[2] == 0) or (m[0] == 0 and m[3] == 0):
+            continue
+        if np.all(m.imag == 0):
+            real_rot += 1
+            for row in (m[:2], m[2:]):
+                assert 1 in (row[0], row[1]) and min(abs(row[0]), abs(row[1])) <= 1 + 1e-15
+    assert real_rot >= 10 and tagged >= 10
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+    want = run(n, ops, psi0)
+    assert np.abs(run(n, f, psi0) - want).max() < 1e-13
+    ff = rotate(n, fuse(n, ops))           # fuse then split, as the planner does
+    assert np.abs(run(n, ff, psi0) - want).max() < 1e-13
+
+
+def test_rotation_split_of_u3_layers():
+    """Random U3 layers with CZ brickwork: every U3 becomes a tagged phase and a real
+    rotation; the list is the same unitary."""
+    n = 8
+    d = hq.DGM()
+    d.setFunction(hq.RandomLayers(n, 6, 11, d))
+    ops = d.ops()
+    f = rotate(n, ops)
+    dense_in = sum(1 for i in range(len(ops)) if not ops[i].control_mask)
+    real_out = sum(1 for i in range(len(f)) if not f[i].control_mask and not f[i].reserved and
+                   _mat(f[i])[1] != 0 and np.all(_mat(f[i]).imag == 0))
+    assert real_out == dense_in
+    psi0 = np.zeros(1 << n, complex)
+    psi0[0] = 1
+    assert np.abs(run(n, f, psi0) - run(n, ops, psi0)).max() < 1e-13
+
+
+def test_rotation_split_leaves_non_unitary_gates_in_the_pivot_form():
+    """A dense gate that is no unitary times a scale has no real middle factor: it keeps
+    the row-scaled complex form, and the list is still the same linear map."""
+    n = 3
+    rng = np.random.default_rng(8)
+    mats = [rng.normal(size=4) + 1j * rng.normal(size=4) for _ in range(6)]
+    ops = hq.make_ops([0, 1, 2, 0, 1, 2], [0] * 6, [0] * 6, mats)
+    f = rotate(n, ops)
+    assert not any(f[i].reserved for i in range(len(f)))
+    psi0 = np.arange(1 << n) + 0.5j
+    want = run(n, ops, psi0)
+    assert np.abs(run(n, f, psi0) - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_rotation_split_rescales_deep_circuits_in_fp32():
+    """2000 Hadamard gates through the rotation split stay finite in complex64."""
+    n = 5
+    h = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+    rng = np.random.default_rng(5)
+    targets = list(rng.integers(0, n, 2000))
+    ops = hq.make_ops(targets, [0] * 2000, [0] * 2000, [h] * 2000)
+    f = rotate(n, ops)
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+    got = apply_list(n, f, psi0, np.complex64)
+    assert np.all(np.isfinite(got))
+    assert np.abs(got - apply_list(n, ops, psi0, np.complex128)).max() < 1e-4
+
+
+def test_rotate_flag_keeps_phases_with_their_rotation():
+    """With HGM_PLAN_ROTATE every tagged phase is planned into the pass of the rotation
+    it precedes, with its target tile-local there."""
+    n = 12
+    d = hq.DGM()
+    d.setFunction(hq.RandomLayers(n, 6, 3, d))
+    ops = d.ops()
+    flags = N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_ROTATE
+    tun = N.default_tuning(block_size=64, repeat_count=4, gpu_region_bits=9, gpu_coalesced_bits=2,
+                           planner=flags, kernel=N.HGM_KERNEL_REG)
+    plan = hq.Plan(n, ops, tun)
+    f = rotate(n, fuse(n, ops))
+    where = {}
+    for p, (local, idx) in enumerate(plan.passes()):
+        for g in idx:
+            where[g] = (p, local)
+    assert len(where) == len(f)
+    for g in range(len(f)):
+        if f[g].reserved:
+            p, local = where[g]
+            assert (local >> f[g].target_bit) & 1
+            assert where[g + 1][0] == p

@@ -125,3 +125,45 @@ def test_random_layers_24q_fp32_bench_tile_vs_oracle():
     err = float(np.abs(s.download() - want).max())
     say("random24 fp32 bench tile: max|gpu - oracle fp64| =", err)
     assert err < 2e-5
+
+
+def test_grover_reference_form_23q(golden_grover23):
+    """The reference's Grover() at qubits = 23 (22-bit register + ancilla, 1608 iterations,
+    src/algorithms/lib_grover.cpp:15-66) through the executor on t_GPU: the bits measured
+    at the reference's seed equal the ones its t_PAR_CPU measured, on the host path with
+    the reference's CLI tuning in FP32 and on the device-resident path in FP64 with the
+    plan-compiled kernels; the register's most probable value is the searched one."""
+    case = golden_grover23
+    n, v, it = case["qubits"], case["search_value"], case["iterations"]
+    assert it == 1608 and case["register_argmax"] == v
+    runs = [(np.complex64, 0, dict(block_size=64, repeat_count=2, gpu_region_bits=8, gpu_coalesced_bits=4)),
+            (np.complex128, 1, dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=4,
+                                    kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE |
+                                    N.HGM_PLAN_PIVOT))]
+    for dtype, resident, tun in runs:
+        d = hq.DGM(dtype)
+        d.qubits = n
+        d.exec_type = hq.T_GPU
+        d.resident = resident
+        for k, val in tun.items():
+            setattr(d, k, val)
+        d.allocateMemory()
+        d.setMemoryValue(1 << (n - 1))
+        d.setFunction(hq.Hadamard(n, 0, n))
+        d.setFunction(hq.GroverIterationRef(n, v), it, reset=False)
+        d.execute(1)
+        st = d.stats
+        say("grover23 ref form", dtype.__name__, "resident" if resident else "host", "passes", st["passes"],
+            "kernel_ms", st["kernel_ms"])
+        if not resident:
+            p = np.abs(d.state.astype(np.complex128)) ** 2
+            marginal = p.reshape(2, -1).sum(axis=0)
+            assert int(np.argmax(marginal)) == v
+            # FP32 states after 145k gates: the reference's CPU rounding and this one drift apart by
+            # up to a few 1e-3 in the marked probability (both near 0.9975-0.9997)
+            assert abs(marginal.max() - case["p_register_max"]) < 5e-3
+        hq.seed_rng(case["seed"])
+        bits = [d.measure(q) for q in range(1, n)]
+        assert bits == case["bits"], (dtype, resident)
+        assert int("".join(map(str, bits)), 2) == v
+        d.freeMemory()

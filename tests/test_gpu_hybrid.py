@@ -327,3 +327,42 @@ def test_reference_engine_golden_circuits_on_libhgm(golden):
             r.set_basis(c["basis"])
             r.execute(exec_type, threads=threads, coalesced_bits=11, region_bits=14, iterations=1)
             assert np.abs(r.state() - ref32).max() < TOL32, (c["name"], exec_type)
+
+
+@pytest.mark.skipif(not po.ref_available(po.REF_HGM_SO),
+                    reason="oracle/_ref/libhgmref_hgm.so not built (needs the reference tree)")
+def test_reference_engine_with_auto_tuning(capfd):
+    """HGM_TUNING=auto (hgm_auto_tuning: plan-compiled kernels, fusion, the rotation split,
+    the measured tile) behind the reference's own engine: general.out at 30 qubits
+    (src/cli/general.cpp:42-52) and Grover at 22 qubits (src/algorithms/lib_grover.cpp:15-66)
+    give the same results as with the reference's CLI tuning; the warm times of both are
+    printed (the first call of each compiles the pass kernels)."""
+    import os
+    import time
+    lib = po._load_ref(po.REF_HGM_SO)
+    times = {}
+    try:
+        for mode in ("cli", "auto"):
+            if mode == "auto":
+                os.environ["HGM_TUNING"] = "auto"
+            else:
+                os.environ.pop("HGM_TUNING", None)
+            for rep in range(2):
+                capfd.readouterr()
+                t0 = time.perf_counter()
+                lib.ref_general_hadamard(30, po.T_GPU, 1)
+                dt = time.perf_counter() - t0
+                line = capfd.readouterr().out.strip().splitlines()[-1]
+                assert abs(float(line.split()[1]) - 2.0 ** -15) < 1e-6, (mode, line)
+                times[("general30", mode, rep)] = dt
+            for rep in range(2):
+                t0 = time.perf_counter()
+                found = lib.ref_grover_run(22, 1234567 % (1 << 21), po.T_GPU, 1, 7 + rep)
+                times[("grover22", mode, rep)] = time.perf_counter() - t0
+                assert found == 1234567 % (1 << 21), (mode, found)
+    finally:
+        os.environ.pop("HGM_TUNING", None)
+    say("reference engine, warm seconds (CLI tuning -> HGM_TUNING=auto):",
+        {w: (round(times[(w, "cli", 1)], 3), round(times[(w, "auto", 1)], 3)) for w in ("general30", "grover22")},
+        "first calls:", {w: (round(times[(w, "cli", 0)], 3), round(times[(w, "auto", 0)], 3))
+                         for w in ("general30", "grover22")})

@@ -196,7 +196,13 @@ def test_random_layers_20q_vs_oracle():
                                  kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FOLD_PHASES),
                 N.default_tuning(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=3,
                                  kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE |
-                                 N.HGM_PLAN_PIVOT)):
+                                 N.HGM_PLAN_PIVOT),
+                N.default_tuning(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=3,
+                                 kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE |
+                                 N.HGM_PLAN_ROTATE | N.HGM_PLAN_BALANCE),
+                N.default_tuning(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4,
+                                 kernel=N.HGM_KERNEL_REG, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE |
+                                 N.HGM_PLAN_ROTATE)):
         s = hq.State(n)
         s.set_basis(0)
         st = s.apply(ops, tun)
@@ -377,7 +383,11 @@ def test_jit_kernel_parity(dtype, tol):
                                 (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT),
                                 (64, 8, 10, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_PIVOT |
                                  N.HGM_PLAN_BALANCE),
-                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_PIVOT | N.HGM_PLAN_LOOKAHEAD)]:
+                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_PIVOT | N.HGM_PLAN_LOOKAHEAD),
+                                (64, 4, 9, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_ROTATE),
+                                (64, 8, 10, 3, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_ROTATE |
+                                 N.HGM_PLAN_BALANCE),
+                                (32, 16, 10, 2, N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE | N.HGM_PLAN_ROTATE)]:
         tun = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=c,
                                kernel=N.HGM_KERNEL_JIT, ops_block=60, planner=planner)
         plan = hq.Plan(n, ops, tun, 1, dtype)
@@ -390,7 +400,8 @@ def test_jit_kernel_parity(dtype, tol):
 
 
 @pytest.mark.parametrize("flag,tile", [(N.HGM_PLAN_FOLD_PHASES, (256, 4, 11, 3)),
-                                       (N.HGM_PLAN_PIVOT | N.HGM_PLAN_BALANCE, (256, 8, 12, 3))])
+                                       (N.HGM_PLAN_PIVOT | N.HGM_PLAN_BALANCE, (256, 8, 12, 3)),
+                                       (N.HGM_PLAN_ROTATE | N.HGM_PLAN_BALANCE, (256, 8, 12, 3))])
 def test_jit_random_layers_30q_round_trip(flag, tile):
     """The benchmark's 30-qubit circuit and configuration: U then U^dagger
     returns |0> and the norm stays 1 (size-independent checks at full size)."""
