@@ -207,6 +207,19 @@ int hgm_state_scatter(const hgm_state *s, void *host, int host_qubits, uint64_t 
 /* Apply a gate list `iterations` times (plans, uploads the op tables once, launches). */
 int hgm_state_apply(hgm_state *s, const hgm_op *ops, int64_t n_ops, const hgm_tuning *tuning,
                     int iterations, hgm_stats *stats);
+/* hgm_state_apply without any device allocation: the plan's tables (register
+ * programs, phase maps, pass tables, or op tables) go to an 8 MiB workspace
+ * that the library holds per GPU for the life of the process, instead of to
+ * buffers it allocates. No counterpart in the reference: for callers that
+ * cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc). Needs a device context (any hgm_state_create or hgm_init call
+ * made it); runs the register or LDS kernels (HGM_KERNEL_JIT is rejected with
+ * HGM_E_TUNING, since loading compiled code allocates); a plan whose tables
+ * exceed 8 MiB is rejected with HGM_E_ARG. Every call on a GPU shares that
+ * GPU's workspace: calls are serialised by the library's lock, and a caller
+ * running other work on the same workspace concurrently would race. */
+int hgm_state_apply_noalloc(hgm_state *s, const hgm_op *ops, int64_t n_ops, const hgm_tuning *tuning,
+                            int iterations, hgm_stats *stats);
 /* Sum of |a|^2 over the whole state (FP64 accumulation on the device). */
 int hgm_state_norm2(const hgm_state *s, double *out);
 /* P(qubit == 1) (FP64 accumulation on the device). qubit counts from the MSB. */

@@ -108,6 +108,8 @@ SIGNATURES = [
     ("hgm_state_copy", C.c_int, [_P, _P]),
     ("hgm_state_apply", C.c_int, [_P, C.POINTER(Op), C.c_int64, C.POINTER(Tuning), C.c_int, C.POINTER(Stats)]),
     ("hgm_state_norm2", C.c_int, [_P, C.POINTER(C.c_double)]),
+    ("hgm_state_apply_noalloc", C.c_int, [_P, C.POINTER(Op), C.c_int64, C.POINTER(Tuning), C.c_int,
+                                          C.POINTER(Stats)]),
     ("hgm_state_prob_one", C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
     ("hgm_state_prob_halves", C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("hgm_state_collapse_reset", C.c_int, [_P, C.c_int, C.c_int, C.c_double]),

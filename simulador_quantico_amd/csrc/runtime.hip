@@ -94,6 +94,16 @@ struct JitModule {
 
 uint64_t g_module_gen = 0;
 
+// Workspace of hgm_state_apply_noalloc: the plan tables (register programs,
+// phase maps and pass tables, or op tables) of one call. Each GPU holds its
+// own copy for the life of the process. In host code `scratch` is only a
+// stand-in for the symbol: its device address is looked up once per device
+// (init_locked) and kept in g_scratch_ptr, and only that address is used.
+constexpr size_t kScratchBytes = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices] = {};
+
 Ctx &ctx() {
     static Ctx c;
     return c;
@@ -145,6 +155,11 @@ int init_locked(int device) {
         HGM_HIP(hipMalloc(&c.pidx, kRedBlocks * sizeof(uint64_t)), "hipMalloc(partials)");
         HGM_HIP(hipHostMalloc(&c.h_partial, 8 * kRedBlocks * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
         HGM_HIP(hipHostMalloc(&c.h_pidx, kRedBlocks * sizeof(uint64_t), hipHostMallocDefault), "hipHostMalloc");
+    }
+    if (device < kMaxDevices && !g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        HGM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[device] = ptr;
     }
     c.device = device;
     c.ready = true;
@@ -271,8 +286,21 @@ int run_plan_graph_locked(hgm_state *s, const hgm_plan *hp, hgm_stats *st) {
     return HGM_OK;
 }
 
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// Bytes of device tables the register or LDS kernels read for plan p.
+size_t plan_table_bytes(const Plan &p) {
+    if (p.kernel == HGM_KERNEL_REG)
+        return align256(p.rops.size() * sizeof(RegOp)) + align256(p.maps.size() * sizeof(PhaseMap)) +
+               align256(p.mems.size() * sizeof(PassMem));
+    return align256(p.ops.size() * sizeof(DevOp));
+}
+
 // Launches every pass of the plan on the context stream. Assumes the lock.
-int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st, double plan_ms) {
+// With `workspace` (at least plan_table_bytes(p) bytes of device memory) the
+// register and LDS kernels read their tables from there instead of from the
+// context's grow-only buffers, and nothing is allocated.
+int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st, double plan_ms, void *workspace = nullptr) {
     Ctx &c = ctx();
     if (s->qubits != p.qubits || s->dtype != p.dtype) {
         t_err = "plan was built for another qubit count or dtype";
@@ -284,24 +312,42 @@ int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st, double plan_ms) 
     JitModule *jm = nullptr;
     if (jit) {
         if ((rc = jit_load_locked(*p.jit, &jm))) return rc;
+    }
+    void *rops = c.rops, *maps = c.maps, *mems = c.mems;
+    DevOp *dops = c.ops;
+    if (workspace && !jit) {
+        char *w = static_cast<char *>(workspace);
+        if (reg) {
+            rops = w;
+            maps = w + align256(p.rops.size() * sizeof(RegOp));
+            mems = static_cast<char *>(maps) + align256(p.maps.size() * sizeof(PhaseMap));
+        } else {
+            dops = reinterpret_cast<DevOp *>(w);
+        }
     } else if (reg) {
         if ((rc = reserve(&c.rops, &c.rops_cap, p.rops.size() * sizeof(RegOp), "hipMalloc(programs)"))) return rc;
         if ((rc = reserve(&c.maps, &c.maps_cap, p.maps.size() * sizeof(PhaseMap), "hipMalloc(maps)"))) return rc;
         if ((rc = reserve(&c.mems, &c.mems_cap, p.mems.size() * sizeof(PassMem), "hipMalloc(pass tables)"))) return rc;
+        rops = c.rops;
+        maps = c.maps;
+        mems = c.mems;
+    } else if (!jit) {
+        if ((rc = reserve_ops(p.ops.size()))) return rc;
+        dops = c.ops;
+    }
+    if (reg) {
         if (!p.rops.empty())
-            HGM_HIP(hipMemcpyAsync(c.rops, p.rops.data(), p.rops.size() * sizeof(RegOp), hipMemcpyHostToDevice,
+            HGM_HIP(hipMemcpyAsync(rops, p.rops.data(), p.rops.size() * sizeof(RegOp), hipMemcpyHostToDevice,
                                    c.stream), "hipMemcpyAsync(programs)");
         if (!p.maps.empty())
-            HGM_HIP(hipMemcpyAsync(c.maps, p.maps.data(), p.maps.size() * sizeof(PhaseMap), hipMemcpyHostToDevice,
+            HGM_HIP(hipMemcpyAsync(maps, p.maps.data(), p.maps.size() * sizeof(PhaseMap), hipMemcpyHostToDevice,
                                    c.stream), "hipMemcpyAsync(maps)");
         if (!p.mems.empty())
-            HGM_HIP(hipMemcpyAsync(c.mems, p.mems.data(), p.mems.size() * sizeof(PassMem), hipMemcpyHostToDevice,
+            HGM_HIP(hipMemcpyAsync(mems, p.mems.data(), p.mems.size() * sizeof(PassMem), hipMemcpyHostToDevice,
                                    c.stream), "hipMemcpyAsync(pass tables)");
-    } else {
-        if ((rc = reserve_ops(p.ops.size()))) return rc;
-        if (!p.ops.empty())
-            HGM_HIP(hipMemcpyAsync(c.ops, p.ops.data(), p.ops.size() * sizeof(DevOp), hipMemcpyHostToDevice, c.stream),
-                    "hipMemcpyAsync(op tables)");
+    } else if (!jit && !p.ops.empty()) {
+        HGM_HIP(hipMemcpyAsync(dops, p.ops.data(), p.ops.size() * sizeof(DevOp), hipMemcpyHostToDevice, c.stream),
+                "hipMemcpyAsync(op tables)");
     }
     PassArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -319,14 +365,14 @@ int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st, double plan_ms) 
         if (jit) {
             if ((rc = launch_jit_pass_locked(s, p, jm, pi))) return rc;
         } else if (reg) {
-            HGM_HIP(launch_pass_reg(p.dtype, 2 * p.shape.R, s->d, static_cast<const RegOp *>(c.rops) + ps.rop_begin,
-                                    (int)ps.rop_count, static_cast<const PhaseMap *>(c.maps),
-                                    static_cast<const PassMem *>(c.mems) + ps.mem, a.nonlocal_mask, a.n_tiles,
+            HGM_HIP(launch_pass_reg(p.dtype, 2 * p.shape.R, s->d, static_cast<const RegOp *>(rops) + ps.rop_begin,
+                                    (int)ps.rop_count, static_cast<const PhaseMap *>(maps),
+                                    static_cast<const PassMem *>(mems) + ps.mem, a.nonlocal_mask, a.n_tiles,
                                     p.shape.r, p.shape.B, ps.phases > 1 ? tile_bytes : 0, c.stream),
                     "pass launch");
         } else {
             std::memcpy(a.local_pos, ps.local_pos, sizeof(a.local_pos));
-            HGM_HIP(launch_pass_lds(p.dtype, s->d, c.ops + ps.op_begin, (int)ps.op_count, a, c.stream), "pass launch");
+            HGM_HIP(launch_pass_lds(p.dtype, s->d, dops + ps.op_begin, (int)ps.op_count, a, c.stream), "pass launch");
         }
     }
     HGM_HIP(hipEventRecord(c.e1, c.stream), "hipEventRecord");
@@ -806,6 +852,40 @@ int hgm_state_apply(hgm_state *s, const hgm_op *ops, int64_t n_ops, const hgm_tu
     if (!s) { t_err = "null state"; return HGM_E_ARG; }
     auto t0 = std::chrono::steady_clock::now();
     int rc = plan_and_run(s, ops, n_ops, tuning, iterations, stats);
+    if (stats) stats->total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+int hgm_state_apply_noalloc(hgm_state *s, const hgm_op *ops, int64_t n_ops, const hgm_tuning *tuning,
+                            int iterations, hgm_stats *stats) {
+    if (!s) { t_err = "null state"; return HGM_E_ARG; }
+    auto t0 = std::chrono::steady_clock::now();
+    hgm_tuning t;
+    if (tuning) t = *tuning; else hgm_default_tuning(&t);
+    if (t.kernel == HGM_KERNEL_JIT) {
+        t_err = "hgm_state_apply_noalloc runs the register or LDS kernels: plan-compiled kernels load a code object";
+        return HGM_E_TUNING;
+    }
+    std::shared_ptr<const Plan> p;
+    double plan_ms = 0;
+    int rc = cached_plan(s->qubits, s->dtype, ops, n_ops, iterations, t, &p, &plan_ms);
+    if (rc) return rc;
+    const size_t need = plan_table_bytes(*p);
+    if (need > kScratchBytes) {
+        t_err = "hgm_state_apply_noalloc: the plan's tables take " + std::to_string(need) + " bytes, above the " +
+                std::to_string(kScratchBytes) + "-byte workspace (fewer passes or gates per call)";
+        return HGM_E_ARG;
+    }
+    Ctx &c = ctx();
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        // a state exists, so the context and the workspace address were set up by its creation
+        if (!c.ready || c.device < 0 || c.device >= kMaxDevices || !g_scratch_ptr[c.device]) {
+            t_err = "hgm_state_apply_noalloc: no device context (hgm_init or hgm_state_create first)";
+            return HGM_E_STATE;
+        }
+        rc = run_plan_locked(s, *p, stats, plan_ms, g_scratch_ptr[c.device]);
+    }
     if (stats) stats->total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
 }
