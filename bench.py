@@ -59,16 +59,17 @@ METRIC = "amp-updates/s + %HBM roofline @30q; circuits/s and p99 latency on serv
 
 WORKLOADS = {
     # name: (qubits, dtype, builder, tuning); planner 96 = greedy + gate fusion + row scaling,
-    # 224 = the same + pass balancing
+    # 224 = the same + pass balancing, 544 = greedy + gate fusion + rotation split, 672 = the same
+    # + pass balancing
     "random30": dict(qubits=30, dtype="c128", kind="random", depth=20, seed=1,
                      tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=3,
-                                 kernel=3, planner=224)),
+                                 kernel=3, planner=672)),
     "u3layers30": dict(qubits=30, dtype="c128", kind="u3layers", depth=20, seed=1,
                        tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=3,
                                    kernel=3, planner=224)),
     "random32f": dict(qubits=32, dtype="c64", kind="random", depth=20, seed=1,
                       tuning=dict(block_size=512, repeat_count=8, gpu_region_bits=13, gpu_coalesced_bits=4,
-                                  kernel=3, planner=96)),
+                                  kernel=3, planner=544)),
     "hadamard20": dict(qubits=20, dtype="c128", kind="hadamard", layers=1,
                        tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4,
                                    planner=96)),
@@ -89,8 +90,9 @@ WORKLOADS = {
 
 def planner_name(flags: int) -> str:
     base = "window" if (flags & 15) == 1 else "greedy"
-    extra = [n for bit, n in ((32, "gate fusion"), (64, "row scaling"), (16, "phase folding"), (128, "pass balancing"),
-                              (256, "two-pass lookahead")) if flags & bit and not (bit == 16 and flags & 64)]
+    extra = [n for bit, n in ((32, "gate fusion"), (512, "rotation split"), (64, "row scaling"), (16, "phase folding"),
+                              (128, "pass balancing"), (256, "two-pass lookahead"))
+             if flags & bit and not (bit == 16 and flags & (64 | 512)) and not (bit == 64 and flags & 512)]
     return " + ".join([base] + extra)
 
 

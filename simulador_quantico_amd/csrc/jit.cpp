@@ -7,6 +7,7 @@
 #include <chrono>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -127,37 +128,71 @@ std::string base_expr(uint64_t nonlocal) {
 
 std::string reg(int k) { return "a" + std::to_string(k); }
 
-void emit_pair(const Gen &g, std::string &o, const RegOp &op, int k0, int k1) {
-    double piv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const double *m = op.m;
+// The 2x2 matrix an op applies to its register pair (row-major, re/im), with
+// the exact 1s of the row-scaled kinds written back in.
+void op_matrix(const RegOp &op, double m[8]) {
     if (op.kind == R_DENSE_PIV || op.kind == R_DENSE_PIV_REAL) {
-        // back to the matrix: the exact 1s and 0s vanish from the generated code
+        std::fill(m, m + 8, 0.0);
         for (int row = 0; row < 2; ++row) {
             const int p = (int)((op.tval >> row) & 1u);
-            piv[4 * row + 2 * p] = 1.0;
-            piv[4 * row + 2 * (1 - p)] = op.m[2 * row];
-            piv[4 * row + 2 * (1 - p) + 1] = op.m[2 * row + 1];
+            m[4 * row + 2 * p] = 1.0;
+            m[4 * row + 2 * (1 - p)] = op.m[2 * row];
+            m[4 * row + 2 * (1 - p) + 1] = op.m[2 * row + 1];
         }
-        m = piv;
+        return;
     }
+    std::copy(op.m, op.m + 8, m);
+}
+
+// One op on the register pair (k0, k1). The generated code keeps each
+// register as its true amplitude divided by a real scale known at generation
+// time (scale[k], 1 unless a gathered phase left one): the op's matrix is
+// applied to the true values, M * diag(s0, s1), and each output row is
+// divided by the scale that makes its largest-weight exact +-1 entry stay
+// exactly 1 (or by 1), which becomes the output register's scale. An op the
+// code applies under a run-time condition keeps each register's scale, so
+// that the registers mean the same whichever branch a thread takes.
+void emit_pair(const Gen &g, std::string &o, const RegOp &op, int k0, int k1, double *scale, bool runtime_cond) {
+    double m[8];
+    op_matrix(op, m);
+    const double s[2] = {scale[k0], scale[k1]};
     o += "        { const " + g.V + " x = " + reg(k0) + ", y = " + reg(k1) + ";\n";
-    if (op.kind == R_SWAP) {
+    if (op.kind == R_SWAP && !runtime_cond) {   // a renaming; the scales follow the registers
         o += "          " + reg(k0) + " = y; " + reg(k1) + " = x; }\n";
+        std::swap(scale[k0], scale[k1]);
         return;
     }
-    if (op.kind == R_ANTI) {
-        o += "          " + reg(k0) + ".x = " + g.lin({{m[2], "y.x"}, {-m[3], "y.y"}}) + ";\n";
-        o += "          " + reg(k0) + ".y = " + g.lin({{m[2], "y.y"}, {m[3], "y.x"}}) + ";\n";
-        o += "          " + reg(k1) + ".x = " + g.lin({{m[4], "x.x"}, {-m[5], "x.y"}}) + ";\n";
-        o += "          " + reg(k1) + ".y = " + g.lin({{m[4], "x.y"}, {m[5], "x.x"}}) + "; }\n";
-        return;
+    double u[2];
+    for (int row = 0; row < 2; ++row) {
+        if (runtime_cond) { u[row] = s[row]; continue; }
+        u[row] = 1.0;
+        for (int col = 0; col < 2; ++col) {
+            const double re = m[4 * row + 2 * col], im = m[4 * row + 2 * col + 1];
+            if (im == 0.0 && (re == 1.0 || re == -1.0)) { u[row] = re * s[col]; break; }
+        }
     }
+    double w[8];
+    for (int row = 0; row < 2; ++row)
+        for (int col = 0; col < 2; ++col)
+            for (int c = 0; c < 2; ++c) w[4 * row + 2 * col + c] = m[4 * row + 2 * col + c] * s[col] / u[row];
+    // exact units stay exact (the division above can leave 1 - 2^-53)
+    for (int row = 0; row < 2; ++row) {
+        for (int col = 0; col < 2; ++col) {
+            const double re = m[4 * row + 2 * col], im = m[4 * row + 2 * col + 1];
+            if (im == 0.0 && (re == 1.0 || re == -1.0) && re * s[col] == u[row]) {
+                w[4 * row + 2 * col] = 1.0;
+                w[4 * row + 2 * col + 1] = 0.0;
+            }
+        }
+    }
+    scale[k0] = u[0];
+    scale[k1] = u[1];
     o += "          " + reg(k0) + ".x = " +
-         g.lin({{m[0], "x.x"}, {-m[1], "x.y"}, {m[2], "y.x"}, {-m[3], "y.y"}}) + ";\n";
-    o += "          " + reg(k0) + ".y = " + g.lin({{m[0], "x.y"}, {m[1], "x.x"}, {m[2], "y.y"}, {m[3], "y.x"}}) + ";\n";
+         g.lin({{w[0], "x.x"}, {-w[1], "x.y"}, {w[2], "y.x"}, {-w[3], "y.y"}}) + ";\n";
+    o += "          " + reg(k0) + ".y = " + g.lin({{w[0], "x.y"}, {w[1], "x.x"}, {w[2], "y.y"}, {w[3], "y.x"}}) + ";\n";
     o += "          " + reg(k1) + ".x = " +
-         g.lin({{m[4], "x.x"}, {-m[5], "x.y"}, {m[6], "y.x"}, {-m[7], "y.y"}}) + ";\n";
-    o += "          " + reg(k1) + ".y = " + g.lin({{m[4], "x.y"}, {m[5], "x.x"}, {m[6], "y.y"}, {m[7], "y.x"}}) +
+         g.lin({{w[4], "x.x"}, {-w[5], "x.y"}, {w[6], "y.x"}, {-w[7], "y.y"}}) + ";\n";
+    o += "          " + reg(k1) + ".y = " + g.lin({{w[4], "x.y"}, {w[5], "x.x"}, {w[6], "y.y"}, {w[7], "y.x"}}) +
          "; }\n";
 }
 
@@ -279,13 +314,44 @@ std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const st
     std::vector<RegOp> ops(plan.rops.begin() + ps.rop_begin, plan.rops.begin() + ps.rop_begin + ps.rop_count);
     schedule_waves(ops);
     std::vector<std::complex<double>> factor((size_t)A, std::complex<double>(1, 0));
+    std::vector<double> scale((size_t)A, 1.0);   // register k holds (true amplitude) / scale[k]
     bool pending = false;
-    // one multiply per register slot for the product of a run of slot phases
+    // The product z of a run of slot phases, once per register slot, as
+    // z = a * (1 + i t) with |t| <= 1 when |Re z| >= |Im z|, else
+    // z = b * (t + i) with b = Im z: two multiply-adds per register, and the
+    // real a or b joins the register's scale.
     auto flush_factors = [&]() {
         if (!pending) return;
-        for (int k = 0; k < A; ++k) emit_factor(g, o, factor[k].real(), factor[k].imag(), k);
+        for (int k = 0; k < A; ++k) {
+            const std::complex<double> z = factor[k];
+            if (z == std::complex<double>(1, 0)) continue;
+            const std::string a = reg(k);
+            if (std::abs(z.real()) >= std::abs(z.imag())) {
+                scale[k] *= z.real();
+                const double t = z.imag() / z.real();
+                if (g.round(t) == 0.0) continue;
+                o += "        { const " + g.V + " x = " + a + ";\n";
+                o += "          " + a + ".x = " + g.lin({{1.0, "x.x"}, {-t, "x.y"}}) + ";\n";
+                o += "          " + a + ".y = " + g.lin({{1.0, "x.y"}, {t, "x.x"}}) + "; }\n";
+            } else {
+                scale[k] *= z.imag();
+                const double t = z.real() / z.imag();
+                o += "        { const " + g.V + " x = " + a + ";\n";
+                o += "          " + a + ".x = " + g.lin({{-1.0, "x.y"}, {t, "x.x"}}) + ";\n";
+                o += "          " + a + ".y = " + g.lin({{1.0, "x.x"}, {t, "x.y"}}) + "; }\n";
+            }
+        }
         std::fill(factor.begin(), factor.end(), std::complex<double>(1, 0));
         pending = false;
+    };
+    // registers back to their true amplitudes (before LDS exchanges and the store)
+    auto flush_scales = [&]() {
+        for (int k = 0; k < A; ++k) {
+            const double sc = g.round(scale[k]);
+            scale[k] = 1.0;
+            if (sc == 1.0) continue;
+            emit_factor(g, o, sc, 0.0, k);
+        }
     };
     for (int64_t i = 0; i < ps.rop_count; ++i) {
         const RegOp &op = ops[i];
@@ -297,6 +363,7 @@ std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const st
         }
         if (!is_phase(op)) flush_factors();   // other phases commute with the pending factors
         if (op.kind == R_REMAP) {
+            flush_scales();
             const int nxt = op.rbit;
             const char *parts[2][2] = {{"", nullptr}, {".x", ".y"}};
             for (int h = 0; h < 2 && parts[g.split][h]; ++h) {
@@ -318,10 +385,11 @@ std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const st
             for (int k = 0; k < A; ++k)
                 if (((uint32_t)k & op.rmask) == op.rval) emit_phase(g, body, op, k);
         } else {
+            const bool runtime_cond = op.tmask || op.gmask;
             for (int k0 = 0; k0 < A; ++k0) {
                 if ((k0 >> op.rbit) & 1) continue;
                 if (((uint32_t)k0 & op.rmask) != op.rval) continue;
-                emit_pair(g, body, op, k0, k0 | (1 << op.rbit));
+                emit_pair(g, body, op, k0, k0 | (1 << op.rbit), scale.data(), runtime_cond);
             }
         }
         if (body.empty()) continue;
@@ -332,6 +400,7 @@ std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const st
         else o += "        if (" + cond + ") {\n" + body + "        }\n";
     }
     flush_factors();
+    flush_scales();
     for (int k = 0; k < A; ++k)
         o += "        s[(base | " + hex64(pm.sreg[k]) + ") + st] = " + reg(k) + ";\n";
     o += "    }\n}\n";

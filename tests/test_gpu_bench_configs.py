@@ -3,10 +3,10 @@ and the closed forms.
 
 * random-layer 28q depth 20 in FP64 with exactly the benchmark's tuning of the
   30-qubit line (tile 256/8/12/3, plan-compiled kernels, greedy planner with
-  fusion, row scaling and pass balancing), against the oracle's FP64 t_PAR_CPU
+  fusion, the rotation split and pass balancing), against the oracle's FP64 t_PAR_CPU
   (src/core/dgm_par_exec.cpp:82-132) at 1e-12 max-abs;
 * random-layer 32q depth 20 in FP32 on the benchmark's FP32 tile (512/8/13/4,
-  plan-compiled, fusion + row scaling): U then U^dagger returns |0>, the norm
+  plan-compiled, fusion + the rotation split): U then U^dagger returns |0>, the norm
   stays 1, and the fidelity against the same circuit run in FP64 on the GPU is
   >= 1 - 1e-6; the same tile and flags at 24 qubits are within 2e-5 of the FP64
   oracle. The reference cannot run 32 qubits (include/core/common.h:31,
@@ -72,7 +72,7 @@ def test_random_layers_28q_fp64_bench_tuning_vs_oracle():
     ops = random_layer_ops(n)
     tun = bench_tuning("random30")
     assert (tun.block_size, tun.repeat_count, tun.gpu_region_bits, tun.gpu_coalesced_bits, tun.kernel,
-            tun.planner) == (256, 8, 12, 3, N.HGM_KERNEL_JIT, 224)
+            tun.planner) == (256, 8, 12, 3, N.HGM_KERNEL_JIT, 672)
     plan = hq.Plan(n, ops, tun)
     s = hq.State(n)
     s.set_basis(0)
@@ -92,7 +92,7 @@ def test_random_layers_32q_fp32_bench_tile():
     ops = random_layer_ops(n)
     tun32 = bench_tuning("random32f")
     assert (tun32.block_size, tun32.repeat_count, tun32.gpu_region_bits, tun32.gpu_coalesced_bits, tun32.kernel,
-            tun32.planner) == (512, 8, 13, 4, N.HGM_KERNEL_JIT, 96)
+            tun32.planner) == (512, 8, 13, 4, N.HGM_KERNEL_JIT, 544)
     s32 = hq.State(n, np.complex64)
     s32.set_basis(0)
     st = hq.Plan(n, ops, tun32, 1, np.complex64).run(s32)
