@@ -26,6 +26,7 @@
 #include <thread>
 #include <vector>
 
+#include "hgm_dgm.hpp"
 #include "hgm_gpu.h"
 #include "kernels.hpp"
 #include "planner.hpp"
@@ -186,30 +187,33 @@ hgm_tuning serving_default_tuning(int n, int dtype) {
     return t;
 }
 
-// A timing circuit for the autotuner: `layers` layers of a dense gate on every
-// qubit followed by a CZ brickwork (the random-layer structure).
-std::vector<hgm_op> timing_circuit(int n, int layers) {
+// The autotuner's timing circuit for `n` qubits: the serving mix's expected
+// work at that qubit count, one request of each kind (the mix draws the kinds
+// uniformly): a Hadamard layer, Grover with 32 iterations (the mean of 1..64),
+// the Draper adder on the low 2*floor((n-1)/2)+1 qubits, and random layers of
+// depth 10, as one gate list.
+std::vector<hgm_op> mix_timing_circuit(int n) {
+    DGM d;
+    d.exit_on_error = false;
+    d.qubits = n;
+    auto add = [&](const std::vector<std::string> &steps, int iterations, std::vector<hgm_op> &ops) {
+        d.setFunction(steps, iterations);
+        std::vector<hgm_op> o = d.ops();
+        ops.insert(ops.end(), o.begin(), o.end());
+    };
     std::vector<hgm_op> ops;
-    const double s = std::sqrt(0.5);
-    for (int l = 0; l < layers; ++l) {
-        for (int q = 0; q < n; ++q) {
-            hgm_op o{};
-            o.target_bit = n - 1 - q;
-            const double ph = 0.37 * (l + 1) + 0.11 * q;
-            const double c = std::cos(ph), sn = std::sin(ph);
-            const double m[8] = {s, 0, s * c, s * sn, s * c, -s * sn, -s, 0};
-            std::memcpy(o.m, m, sizeof m);
-            ops.push_back(o);
-        }
-        for (int q = l & 1; q + 1 < n; q += 2) {
-            hgm_op o{};
-            o.target_bit = n - 2 - q;
-            o.control_mask = o.control_value = uint64_t(1) << (n - 1 - q);
-            const double m[8] = {1, 0, 0, 0, 0, 0, -1, 0};
-            std::memcpy(o.m, m, sizeof m);
-            ops.push_back(o);
-        }
+    add({Hadamard(n, 0, n)}, 1, ops);
+    add(GroverIteration(n, (uint64_t(0x5a5a5a5a5a5a5a5aull) & ((uint64_t(1) << n) - 1))), 32, ops);
+    const int w = (n - 1) / 2;
+    if (w >= 1) {
+        DGM da;
+        da.exit_on_error = false;
+        da.qubits = 2 * w + 1;
+        da.setFunction(DraperAdder(w, da.gates));
+        std::vector<hgm_op> o = da.ops();
+        ops.insert(ops.end(), o.begin(), o.end());
     }
+    add(RandomLayers(n, 10, 2026, d.gates), 1, ops);
     return ops;
 }
 
@@ -595,7 +599,13 @@ int hgm_server_create(const hgm_server_config *cfg, hgm_server **out) {
     for (int n = 1; n <= kMaxQubits; ++n) s->tune[n] = serving_default_tuning(n, cfg->dtype);
     if (cfg->autotune) {
         for (int n = std::max(cfg->min_qubits, 14); n <= cfg->max_qubits; ++n) {
-            const std::vector<hgm_op> ops = timing_circuit(n, 4);
+            std::vector<hgm_op> ops;
+            try {
+                ops = mix_timing_circuit(n);
+            } catch (const std::exception &e) {
+                set_error(std::string("serving autotune: ") + e.what());
+                return HGM_E_ARG;
+            }
             hgm_tuning best;
             double ms;
             if ((rc = hgm_autotune(n, cfg->dtype, ops.data(), (int64_t)ops.size(), 1, HGM_KERNEL_REG,
