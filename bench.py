@@ -642,6 +642,12 @@ def main():
         out["roofline"]["compute"].update(achieved=executed, frac=executed / fpeak,
                                           executed_flops_per_step=valu["executed_fp64_flops_per_step"],
                                           executed_source=valu["source"])
+        if "floors_ms" in valu:
+            # per pass, the VALU issue floor at the measured clock and the HBM floor at the copy rate, summed
+            # over the step: the larger sum names the resource that bounds the step
+            f = valu["floors_ms"]
+            out["roofline"]["floors_ms_per_step"] = dict(f, source=valu["source"])
+            out["roofline"]["bound"] = "hbm" if f["hbm_at_copy_rate_ms"] >= f["valu_ms"] else "fp64-valu"
     default_run = args.tuning is None and args.kernel is None and args.planner is None and args.ops_block is None
     # the copy ceiling beside the published peak: one streaming kernel that reads and writes the
     # whole state (hgm_state_copy), median of 5 after one untimed copy
