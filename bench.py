@@ -60,10 +60,10 @@ METRIC = "amp-updates/s + %HBM roofline @30q; circuits/s and p99 latency on serv
 WORKLOADS = {
     # name: (qubits, dtype, builder, tuning); planner 96 = greedy + gate fusion + row scaling,
     # 224 = the same + pass balancing, 544 = greedy + gate fusion + rotation split, 672 = the same
-    # + pass balancing
+    # + pass balancing, 928 = the same + two-pass lookahead
     "random30": dict(qubits=30, dtype="c128", kind="random", depth=20, seed=1,
-                     tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=3,
-                                 kernel=3, planner=672)),
+                     tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4,
+                                 kernel=3, planner=928)),
     "u3layers30": dict(qubits=30, dtype="c128", kind="u3layers", depth=20, seed=1,
                        tuning=dict(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=3,
                                    kernel=3, planner=224)),

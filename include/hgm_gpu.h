@@ -289,8 +289,8 @@ int hgm_rotate_rows(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, i
 /* The fast tuning of this library for a state of `qubits` qubits: the
  * plan-compiled kernels, gate fusion, the rotation split and pass balancing on
  * the greedy planner, and the tile measured fastest on the 30- and 32-qubit
- * random-layer circuits (FP64: 256 threads x 8 pairs on 2^12 tiles, 3
- * coalesced bits; FP32: 512 x 8 on 2^13, 4 coalesced bits). No counterpart in
+ * random-layer circuits (FP64: 256 threads x 8 pairs on 2^12 tiles; FP32:
+ * 512 x 8 on 2^13; 4 coalesced bits). No counterpart in
  * the reference, whose tuning comes from the command line
  * (include/cli/cli_common.h:21-25). Callers of the reference's ABI
  * (GpuExecutionWrapper) and of the executor opt in with the environment
@@ -309,8 +309,8 @@ int hgm_validate_tuning(int qubits, int dtype, const hgm_tuning *t);
  * and returns the fastest tuning and its kernel time. Shapes: 2*repeat_count
  * in {8, 16, 32} amplitudes per thread; for AUTO, REG and LDS gpu_region_bits
  * 8..13 and gpu_coalesced_bits 2..4; for JIT, whose every candidate is
- * compiled, gpu_region_bits 10..13 and the coalesced bits that make 128-byte
- * runs (3 for FP64, 4 for FP32). No counterpart in the reference, whose tuning
+ * compiled, gpu_region_bits 10..13 and the coalesced bits that make 128- and
+ * 256-byte runs (3 and 4 for FP64, 4 and 5 for FP32). No counterpart in the reference, whose tuning
  * is given by hand on the command line (include/cli/cli_common.h:21-25). */
 int hgm_autotune(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, int kernel, int planner,
                  hgm_tuning *best, double *best_ms);

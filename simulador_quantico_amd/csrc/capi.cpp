@@ -482,7 +482,7 @@ int hgm_auto_tuning(int qubits, int dtype, hgm_tuning *out) {
     out->block_size = f32 ? 512 : 256;
     out->repeat_count = 8;
     out->gpu_region_bits = f32 ? 13 : 12;
-    out->gpu_coalesced_bits = f32 ? 4 : 3;
+    out->gpu_coalesced_bits = 4;
     out->kernel = HGM_KERNEL_JIT;
     out->planner = HGM_PLAN_GREEDY | HGM_PLAN_FUSE | hgm::kAutoRowFlag | HGM_PLAN_BALANCE;
     return HGM_OK;

@@ -99,11 +99,22 @@ double gate_cost(const hgm_op &o) {
     return ops / double(uint64_t(1) << __builtin_popcountll(o.control_mask));
 }
 
-// Moves gates between neighbouring passes, from the one with more arithmetic
-// to the one with less, while that lowers the larger of the two: a gate
-// moves to the front of the next pass when it commutes with every gate after
-// it in its own pass, or to the end of the previous pass when it commutes
-// with every gate before it, and only where its target is tile-local; a
+// Arithmetic (gate_cost units: multiply-adds per amplitude pair) that one pass
+// over the state hides under its memory sweep on MI355X: a wave64 FP64 FMA
+// issues every 4 cycles on each of the 1024 SIMDs (about 3e13 pair-ops/s at
+// the loaded clock), and a sweep moves 64 bytes per FP64 pair at about
+// 6.1 TB/s (1e11 pairs/s); FP32 halves both. Measured on the 30-qubit
+// random-layer circuit, the per-pass VALU floors and memory floors sum to
+// about the same time at ~300 per pass.
+constexpr double kSweepCost = 300.0;
+
+// Moves gates between passes up to four apart, from the one with more
+// arithmetic to the one with less, while that lowers the sum of the two
+// passes' times (max of sweep and arithmetic) or, at equal sums, the larger of
+// them: a gate moves to the front of a later pass when it commutes with every
+// gate after it in its own pass and with every gate of the passes in between,
+// or to the end of an earlier pass likewise, and only where its target is
+// tile-local there; a
 // phase that rides with the rotation after it (HGM_PLAN_ROTATE) moves with
 // that rotation. The pass count and the set of gates are unchanged.
 void balance_passes(const std::vector<GateInfo> &info, const hgm_op *ops, int64_t n_ops, int64_t cap,
@@ -128,72 +139,100 @@ void balance_passes(const std::vector<GateInfo> &info, const hgm_op *ops, int64_
     std::vector<double> cost(P, 0.0);
     for (size_t i = 0; i < P; ++i)
         for (int64_t k : lists[i]) cost[i] += gate_cost(ops[k % n_ops]);
-    for (int sweep = 0; sweep < 8; ++sweep) {
+    // A pass takes about max(memory sweep, arithmetic): arithmetic below the
+    // sweep is hidden. Moving cost c from pass a to pass b helps when it
+    // lowers the sum of the two, or, at equal sums, the larger of the two.
+    auto eff = [&](double x) { return std::max(x, kSweepCost); };
+    auto helps = [&](double from, double to, double c) {
+        const double gain = eff(from) + eff(to) - eff(from - c) - eff(to + c);
+        return c > 0 && (gain > 1e-9 || (gain > -1e-9 && c < from - to));
+    };
+    // the writes and reads of all the gates of a pass
+    std::vector<uint64_t> pw(P, 0), pr(P, 0);
+    auto refresh = [&](size_t i) {
+        pw[i] = pr[i] = 0;
+        for (int64_t k : lists[i]) { pw[i] |= info[k % n_ops].writes; pr[i] |= info[k % n_ops].reads; }
+    };
+    for (size_t i = 0; i < P; ++i) refresh(i);
+    constexpr size_t kHops = 4;   // a gate moves at most this many passes
+    for (int sweep = 0; sweep < 16; ++sweep) {
         bool moved = false;
-        for (size_t i = 0; i + 1 < P; ++i) {
-            // forward: the tail of pass i to the front of pass i+1
-            while (cost[i] > cost[i + 1] && (int64_t)lists[i + 1].size() < cap) {
-                const double gap = cost[i] - cost[i + 1];
+        for (size_t i = 0; i < P; ++i) {
+            // forward: a unit at the tail of pass i (commuting with the gates after it there) to the
+            // front of a later pass j, when it commutes with every gate of the passes in between
+            for (bool again = true; again;) {
+                again = false;
                 uint64_t bw = 0, br = 0;
-                int64_t pa = -1, pb = -1;
-                for (int64_t j = (int64_t)lists[i].size() - 1; j >= 0; --j) {
-                    const int64_t a = unit_begin(lists[i], j);
+                for (int64_t jj = (int64_t)lists[i].size() - 1; jj >= 0 && !again; --jj) {
+                    const int64_t a = unit_begin(lists[i], jj);
                     uint64_t uw = 0, ur = 0, need = 0;
-                    for (int64_t k = a; k <= j; ++k) {
+                    for (int64_t k = a; k <= jj; ++k) {
                         const GateInfo &g = info[lists[i][k] % n_ops];
                         uw |= g.writes;
                         ur |= g.reads;
                         need |= g.need;
                     }
-                    const double c = cost_of(lists[i], a, j + 1);
-                    if (!(uw & (bw | br)) && !(ur & bw) && !(need & ~local[i + 1]) && c > 0 && c < gap) {
-                        pa = a;
-                        pb = j + 1;
-                        break;
+                    const double c = cost_of(lists[i], a, jj + 1);
+                    if (!(uw & (bw | br)) && !(ur & bw)) {
+                        uint64_t mw = 0, mr = 0;
+                        for (size_t j = i + 1; j < P && j <= i + kHops; ++j) {
+                            if ((int64_t)lists[j].size() < cap && !(need & ~local[j]) && helps(cost[i], cost[j], c)) {
+                                lists[j].insert(lists[j].begin(), lists[i].begin() + a, lists[i].begin() + jj + 1);
+                                lists[i].erase(lists[i].begin() + a, lists[i].begin() + jj + 1);
+                                cost[i] -= c;
+                                cost[j] += c;
+                                refresh(i);
+                                refresh(j);
+                                moved = again = true;
+                                break;
+                            }
+                            mw |= pw[j];
+                            mr |= pr[j];
+                            if ((uw & (mw | mr)) || (ur & mw)) break;   // blocked by the gates of pass j
+                        }
                     }
                     bw |= uw;
                     br |= ur;
-                    j = a;
+                    jj = a;
                 }
-                if (pa < 0) break;
-                const double c = cost_of(lists[i], pa, pb);
-                lists[i + 1].insert(lists[i + 1].begin(), lists[i].begin() + pa, lists[i].begin() + pb);
-                lists[i].erase(lists[i].begin() + pa, lists[i].begin() + pb);
-                cost[i] -= c;
-                cost[i + 1] += c;
-                moved = true;
             }
-            // backward: the head of pass i+1 to the end of pass i
-            while (cost[i + 1] > cost[i] && (int64_t)lists[i].size() < cap) {
-                const double gap = cost[i + 1] - cost[i];
+            // backward: a unit at the head of pass i to the end of an earlier pass j
+            for (bool again = true; again;) {
+                again = false;
                 uint64_t bw = 0, br = 0;
-                int64_t pa = -1, pb = -1;
-                for (int64_t j = 0; j < (int64_t)lists[i + 1].size();) {
-                    const int64_t e = unit_end(lists[i + 1], j);
+                for (int64_t jj = 0; jj < (int64_t)lists[i].size() && !again;) {
+                    const int64_t e = unit_end(lists[i], jj);
                     uint64_t uw = 0, ur = 0, need = 0;
-                    for (int64_t k = j; k < e; ++k) {
-                        const GateInfo &g = info[lists[i + 1][k] % n_ops];
+                    for (int64_t k = jj; k < e; ++k) {
+                        const GateInfo &g = info[lists[i][k] % n_ops];
                         uw |= g.writes;
                         ur |= g.reads;
                         need |= g.need;
                     }
-                    const double c = cost_of(lists[i + 1], j, e);
-                    if (!(uw & (bw | br)) && !(ur & bw) && !(need & ~local[i]) && c > 0 && c < gap) {
-                        pa = j;
-                        pb = e;
-                        break;
+                    const double c = cost_of(lists[i], jj, e);
+                    if (!(uw & (bw | br)) && !(ur & bw)) {
+                        uint64_t mw = 0, mr = 0;
+                        for (size_t d = 1; d <= kHops && d <= i; ++d) {
+                            const size_t j = i - d;
+                            if ((int64_t)lists[j].size() < cap && !(need & ~local[j]) && helps(cost[i], cost[j], c)) {
+                                lists[j].insert(lists[j].end(), lists[i].begin() + jj, lists[i].begin() + e);
+                                lists[i].erase(lists[i].begin() + jj, lists[i].begin() + e);
+                                cost[i] -= c;
+                                cost[j] += c;
+                                refresh(i);
+                                refresh(j);
+                                moved = again = true;
+                                break;
+                            }
+                            mw |= pw[j];
+                            mr |= pr[j];
+                            if ((uw & (mw | mr)) || (ur & mw)) break;
+                        }
                     }
                     bw |= uw;
                     br |= ur;
-                    j = e;
+                    jj = e;
                 }
-                if (pa < 0) break;
-                const double c = cost_of(lists[i + 1], pa, pb);
-                lists[i].insert(lists[i].end(), lists[i + 1].begin() + pa, lists[i + 1].begin() + pb);
-                lists[i + 1].erase(lists[i + 1].begin() + pa, lists[i + 1].begin() + pb);
-                cost[i + 1] -= c;
-                cost[i] += c;
-                moved = true;
             }
         }
         if (!moved) break;

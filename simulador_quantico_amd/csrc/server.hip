@@ -752,7 +752,8 @@ int hgm_autotune(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int it
         for (int r = r_lo; r <= 13; ++r) {
             const int64_t B = (int64_t(1) << r) / A;
             if (B < 64 || B > 1024 || (kernel != HGM_KERNEL_LDS && B > reg_kernel_max_threads(A))) continue;
-            for (int c = jit ? c_run : 2; c <= (jit ? c_run : 4); ++c) {
+            // plan-compiled candidates: runs of 128 and 256 bytes per wave segment
+            for (int c = jit ? c_run : 2; c <= (jit ? c_run + 1 : 4); ++c) {
                 hgm_tuning t;
                 hgm_default_tuning(&t);
                 t.block_size = (int)B;

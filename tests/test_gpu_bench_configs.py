@@ -2,8 +2,8 @@
 and the closed forms.
 
 * random-layer 28q depth 20 in FP64 with exactly the benchmark's tuning of the
-  30-qubit line (tile 256/8/12/3, plan-compiled kernels, greedy planner with
-  fusion, the rotation split and pass balancing), against the oracle's FP64 t_PAR_CPU
+  30-qubit line (tile 256/8/12/4, plan-compiled kernels, greedy planner with
+  fusion, the rotation split, pass balancing and the two-pass lookahead), against the oracle's FP64 t_PAR_CPU
   (src/core/dgm_par_exec.cpp:82-132) at 1e-12 max-abs;
 * random-layer 32q depth 20 in FP32 on the benchmark's FP32 tile (512/8/13/4,
   plan-compiled, fusion + the rotation split): U then U^dagger returns |0>, the norm
@@ -72,7 +72,7 @@ def test_random_layers_28q_fp64_bench_tuning_vs_oracle():
     ops = random_layer_ops(n)
     tun = bench_tuning("random30")
     assert (tun.block_size, tun.repeat_count, tun.gpu_region_bits, tun.gpu_coalesced_bits, tun.kernel,
-            tun.planner) == (256, 8, 12, 3, N.HGM_KERNEL_JIT, 672)
+            tun.planner) == (256, 8, 12, 4, N.HGM_KERNEL_JIT, 928)
     plan = hq.Plan(n, ops, tun)
     s = hq.State(n)
     s.set_basis(0)
