@@ -130,7 +130,8 @@ def test_autotune_plan_compiled_family():
     ms = C.c_double()
     N.check(N.lib.hgm_autotune(n, N.HGM_C128, ops, len(ops), 1, N.HGM_KERNEL_JIT, flags, C.byref(best), C.byref(ms)))
     say("autotune jit", best.block_size, best.repeat_count, best.gpu_region_bits, best.gpu_coalesced_bits, ms.value)
-    assert best.kernel == N.HGM_KERNEL_JIT and best.planner == flags and best.gpu_coalesced_bits == 3
+    # FP64 candidates have 128-byte (c=3) or 256-byte (c=4) runs per wave segment
+    assert best.kernel == N.HGM_KERNEL_JIT and best.planner == flags and best.gpu_coalesced_bits in (3, 4)
     assert 10 <= best.gpu_region_bits <= 13
     s = hq.State(n)
     s.set_basis(0)
