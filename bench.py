@@ -384,6 +384,12 @@ def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026,
     by_job = {r["job"]: r for r in res}
     adders = [j for j in jobs if j.kind == "adder"]
     adder_ok = sum(by_job[j.index]["argmax"] == j.expect for j in adders)
+    # state traffic of the saturated run: every pass reads and writes the whole FP32 state
+    traffic_by_kind = {}
+    for r in res:
+        j = jobs[r["job"]]
+        traffic_by_kind[j.kind] = traffic_by_kind.get(j.kind, 0) + r["passes"] * (1 << j.qubits) * 2 * 8
+    traffic = sum(traffic_by_kind.values())
     print(f"bench: served {len(res)} in {(t_last - t_first) / 1e3:.2f} s", file=sys.stderr, flush=True)
 
     S.run_sequential(warm[:4])
@@ -413,6 +419,9 @@ def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026,
                             "p99_latency_ms": float(np.percentile(seq_lat, 99)),
                             "mean_service_ms": float(service.mean()), "argmax_agree": agree},
         "speedup_circuits_per_s": served_cps / seq_cps,
+        "state_traffic": {"bytes": traffic, "achieved_TBps": traffic / ((t_last - t_first) / 1e3) / 1e12,
+                          "share_by_kind": {k: v / traffic for k, v in sorted(traffic_by_kind.items())},
+                          "counting": "passes x 2^qubits x 8 B, read and written once per pass"},
         "at_0.8_capacity": {
             "arrival_rate_per_s": below_rate, "requests": len(res_below),
             "p50_latency_ms": float(np.percentile([r["latency_ms"] for r in res_below], 50)),
