@@ -18,11 +18,12 @@ namespace hgm {
 
 namespace {
 
-uint64_t fnv1a(const std::string &s) {
+uint64_t fnv1a(const char *p, size_t n) {
     uint64_t h = 1469598103934665603ull;
-    for (unsigned char c : s) { h ^= c; h *= 1099511628211ull; }
+    for (size_t i = 0; i < n; ++i) { h ^= (unsigned char)p[i]; h *= 1099511628211ull; }
     return h;
 }
+uint64_t fnv1a(const std::string &s) { return fnv1a(s.data(), s.size()); }
 
 std::string hex64(uint64_t v) {
     char b[32];
@@ -432,11 +433,18 @@ std::string pass_key(const Plan &plan, const Pass &ps) {
 struct Compiled {
     std::string source;          // compared on lookup: the hash only narrows the search
     std::vector<char> binary;
+    uint64_t binary_hash;
     double seconds;
+    uint64_t last_use;
 };
 
+// Compiled code objects by source hash, least recently used first out beyond
+// kMaxCompiled (an autotune sweep or a stream of distinct circuits would
+// otherwise keep every source and binary for the life of the process).
+constexpr size_t kMaxCompiled = 64;
 std::mutex g_cache_mu;
 std::unordered_map<uint64_t, std::shared_ptr<Compiled>> g_cache;
+uint64_t g_cache_tick = 0;
 
 }  // namespace
 
@@ -489,7 +497,9 @@ int jit_compile(JitCode *out, std::string *err) {
         auto it = g_cache.find(out->hash);
         if (it != g_cache.end() && it->second->source == out->source) {
             out->binary = it->second->binary;
+            out->binary_hash = it->second->binary_hash;
             out->compile_s = 0;
+            it->second->last_use = ++g_cache_tick;
             return HGM_OK;
         }
     }
@@ -517,14 +527,28 @@ int jit_compile(JitCode *out, std::string *err) {
     out->binary.assign(n, 0);
     hiprtcGetCode(prog, out->binary.data());
     hiprtcDestroyProgram(&prog);
+    out->binary_hash = fnv1a(out->binary.data(), out->binary.size());
     out->compile_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     auto c = std::make_shared<Compiled>();
     c->source = out->source;
     c->binary = out->binary;
+    c->binary_hash = out->binary_hash;
     c->seconds = out->compile_s;
     std::lock_guard<std::mutex> lk(g_cache_mu);
+    c->last_use = ++g_cache_tick;
     g_cache[out->hash] = c;
+    while (g_cache.size() > kMaxCompiled) {
+        auto lru = g_cache.begin();
+        for (auto it = g_cache.begin(); it != g_cache.end(); ++it)
+            if (it->second->last_use < lru->second->last_use) lru = it;
+        g_cache.erase(lru);
+    }
     return HGM_OK;
+}
+
+size_t jit_compiled_count() {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    return g_cache.size();
 }
 
 }  // namespace hgm

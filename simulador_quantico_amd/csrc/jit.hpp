@@ -19,6 +19,7 @@ namespace hgm {
 struct JitCode {
     std::string source;          // generated HIP source of all pass kernels
     std::vector<char> binary;    // gfx950 code object
+    uint64_t binary_hash = 0;    // of the code object
     std::vector<int32_t> pass_kernel;   // kernel index of each pass (passes with identical code share one)
     std::vector<std::string> names;     // kernel symbol names
     size_t lds_bytes = 0;        // dynamic LDS of a pass kernel with remaps
@@ -30,7 +31,10 @@ struct JitCode {
 void jit_generate(const Plan &plan, JitCode *out);
 
 // Compiles out->source with hipRTC for gfx950 (cached by source hash within
-// the process). Returns an HGM_* status.
+// the process, the 64 most recently used). Returns an HGM_* status.
 int jit_compile(JitCode *out, std::string *err);
+
+// Number of code objects in the compile cache.
+size_t jit_compiled_count();
 
 }  // namespace hgm

@@ -88,9 +88,18 @@ struct Ctx {
 struct JitModule {
     hipModule_t mod = nullptr;
     std::vector<hipFunction_t> fns;
-    std::vector<char> binary;   // compared on lookup: the source hash only narrows the search
+    size_t binary_size = 0;     // with binary_hash, compared on lookup: the source hash only narrows the search
+    uint64_t binary_hash = 0;
     uint64_t gen = 0;           // load generation: graphs captured with an older one are stale
+    uint64_t last_use = 0;
 };
+
+// Loaded plan-compiled modules, least recently used first out beyond
+// kMaxJitModules (unloaded after the library's stream drains; a plan whose
+// module went out is loaded again on its next run, and its captured graph,
+// tagged with the old load generation, is captured again).
+constexpr size_t kMaxJitModules = 16;
+uint64_t g_module_tick = 0;
 
 uint64_t g_module_gen = 0;
 
@@ -170,23 +179,39 @@ int init_locked(int device) {
 int jit_load_locked(const JitCode &jc, JitModule **out) {
     auto &mods = jit_modules();
     auto it = mods.find(jc.hash);
-    if (it != mods.end() && it->second.binary != jc.binary) {   // another plan's code under the same hash
+    if (it != mods.end() && (it->second.binary_size != jc.binary.size() || it->second.binary_hash != jc.binary_hash)) {
+        // another plan's code under the same source hash
+        hipStreamSynchronize(ctx().stream);
         hipModuleUnload(it->second.mod);
         mods.erase(it);
         it = mods.end();
     }
     if (it == mods.end()) {
+        while (mods.size() >= kMaxJitModules) {
+            auto lru = mods.begin();
+            for (auto m = mods.begin(); m != mods.end(); ++m)
+                if (m->second.last_use < lru->second.last_use) lru = m;
+            hipStreamSynchronize(ctx().stream);   // no kernel of it is still queued
+            hipModuleUnload(lru->second.mod);
+            mods.erase(lru);
+        }
         JitModule m;
         HGM_HIP(hipModuleLoadData(&m.mod, jc.binary.data()), "hipModuleLoadData(pass kernels)");
         for (const std::string &name : jc.names) {
             hipFunction_t f;
-            HGM_HIP(hipModuleGetFunction(&f, m.mod, name.c_str()), "hipModuleGetFunction");
+            const hipError_t e = hipModuleGetFunction(&f, m.mod, name.c_str());
+            if (e != hipSuccess) {
+                hipModuleUnload(m.mod);
+                return hip_fail(e, "hipModuleGetFunction");
+            }
             m.fns.push_back(f);
         }
-        m.binary = jc.binary;
+        m.binary_size = jc.binary.size();
+        m.binary_hash = jc.binary_hash;
         m.gen = ++g_module_gen;
         it = mods.emplace(jc.hash, std::move(m)).first;
     }
+    it->second.last_use = ++g_module_tick;
     *out = &it->second;
     return HGM_OK;
 }

@@ -454,6 +454,31 @@ def test_jit_graph_replay_matches_direct_launches():
                                          kernel=N.HGM_KERNEL_REG)).set_graph(True)
 
 
+def test_jit_modules_beyond_the_cache_bound():
+    """More distinct plan-compiled plans than the library keeps loaded (16):
+    each still matches the oracle when run again after its module went out,
+    and a graph captured before the eviction is captured again, not replayed
+    with unloaded code."""
+    n = 12
+    rng = np.random.default_rng(91)
+    tun = N.default_tuning(block_size=64, repeat_count=4, gpu_region_bits=9, gpu_coalesced_bits=3,
+                           kernel=N.HGM_KERNEL_JIT)
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+    cases = []
+    for _ in range(18):
+        t, cm, cv, mats = _random_gate_list(rng, n, 24)
+        cases.append((hq.Plan(n, hq.make_ops(t, cm, cv, mats), tun),
+                      oracle_run_gates(n, t, cm, cv, mats, state=psi0)))
+    cases[0][0].set_graph(True)
+    for rep in range(2):
+        for k, (plan, want) in enumerate(cases):
+            s = hq.State(n)
+            s.upload(psi0)
+            plan.run(s)
+            assert np.abs(s.download() - want).max() < TOL64, (rep, k)
+
+
 def test_one_call_entry_reuses_plans_only_for_the_same_circuit():
     """hgm_execute keeps recent plans: the same gate list runs again without
     planning, and a list that differs in one matrix entry gets its own plan."""
