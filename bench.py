@@ -340,7 +340,7 @@ def shor_attempts(number: int, attempts: int, dtype, resident: bool, tuning: dic
 
 
 def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026, autotune: bool = False,
-                  fidelity_sample: int = 24, tile=None, planner=None) -> dict:
+                  fidelity_sample: int = 24, tile=None, planner=None, compiled: bool = True) -> dict:
     """The serving mix on the batched server (FP32 states) against sequential
     plain FP64 one-circuit-per-call runs of the same request stream."""
     from simulador_quantico_amd import _native as N
@@ -358,6 +358,15 @@ def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026,
         if planner is not None:   # another gate-list rewrite for every qubit count of the mix
             for n in range(10, 27):
                 srv.set_tuning(n, planner=planner)
+        pre = {"kernels": 0, "seconds": 0.0}
+        if compiled:   # pass kernels of the mix's circuit shapes, compiled before the timed run
+            t_pc = time.perf_counter()
+            for n in range(14, 27):
+                for ops, iters in S.mix_shapes(n):
+                    pre["kernels"] += srv.precompile(n, ops, iters)
+            pre["seconds"] = time.perf_counter() - t_pc
+            print(f"bench: serving precompiled {pre['kernels']} pass kernels in {pre['seconds']:.1f} s",
+                  file=sys.stderr, flush=True)
         S.run_served(warm, srv, time_scale=0.0)
         res = S.run_served(jobs, srv)
         t_sat = (max(r["t_done_ms"] for r in res) - min(r["t_submit_ms"] for r in res)) / 1e3
@@ -419,6 +428,9 @@ def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026,
                             "p99_latency_ms": float(np.percentile(seq_lat, 99)),
                             "mean_service_ms": float(service.mean()), "argmax_agree": agree},
         "speedup_circuits_per_s": served_cps / seq_cps,
+        "compiled_pass_kernels": dict(pre, note="one kernel per pass structure of the mix's circuit shapes at 14-26 "
+                                      "qubits, compiled before the timed run; states of at most 13 qubits and "
+                                      "passes of other structures run the batched interpreter"),
         "state_traffic": {"bytes": traffic, "achieved_TBps": traffic / ((t_last - t_first) / 1e3) / 1e12,
                           "share_by_kind": {k: v / traffic for k, v in sorted(traffic_by_kind.items())},
                           "counting": "passes x 2^qubits x 8 B, read and written once per pass"},
@@ -465,6 +477,8 @@ def main():
     ap.add_argument("--autotune-tile", action="store_true",
                     help="autotune the workload's tile shape (hgm_autotune, untimed) before the timed steps")
     ap.add_argument("--serving-tile", default=None, help="serving: B,R,r,c for states of 14 qubits and more")
+    ap.add_argument("--interpreted-serving", action="store_true",
+                    help="serving: no compiled pass kernels (every pass on the batched interpreter)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg of the default run")
     ap.add_argument("--general-out", type=int, default=None, metavar="QUBITS",
                     help="time the reference's general.out (H x3, FP32, t_GPU, host state) at QUBITS qubits")
@@ -532,7 +546,7 @@ def main():
     if args.serving:
         tile = tuple(int(x) for x in args.serving_tile.split(",")) if args.serving_tile else None
         out = serving_bench(args.requests, args.rate, args.seq_sample, autotune=args.autotune, tile=tile,
-                            planner=args.planner)
+                            planner=args.planner, compiled=not args.interpreted_serving)
         print(json.dumps({"metric": METRIC, "value": out["circuits_per_s"], "unit": "circuits/s", "n_gpus": 1,
                           "higher_is_better": True, "vs_baseline": None, "dtype": "f32", "data": "synthetic",
                           "config": {"workload": "serving mix"}, "serving": out}))
@@ -685,7 +699,7 @@ def main():
     if default_run and not args.no_serving:
         try:
             out["serving"] = serving_bench(args.requests, args.rate, min(args.seq_sample, 128),
-                                           autotune=args.autotune)
+                                           autotune=args.autotune, compiled=not args.interpreted_serving)
         except Exception as e:
             out["serving"] = {"error": str(e)}
     if default_run and not args.no_configs:

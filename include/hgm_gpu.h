@@ -265,6 +265,12 @@ int hgm_plan_maps(const hgm_plan *p, void *maps, int64_t *n_maps);
 /* Generated source of an HGM_KERNEL_JIT plan (empty for other kernels); out
  * may be NULL to query the length. */
 int hgm_plan_jit_source(const hgm_plan *p, char *out, size_t cap, size_t *len);
+/* Source of the serving engine's compiled kernel for pass `pass` of an
+ * HGM_KERNEL_REG plan (see hgm_server_precompile): a function of the pass's
+ * structure only, so plans of circuits that differ in matrix entries or
+ * control values have equal sources. out may be NULL to query the length.
+ * No counterpart in the reference (an inspection aid of this library). */
+int hgm_plan_serving_source(const hgm_plan *p, int64_t pass, char *out, size_t cap, size_t *len);
 /* Replay the plan's passes from a hipGraph (HGM_KERNEL_JIT plans only): the
  * first hgm_state_run_plan on a state captures its launches, later runs on
  * the same state replay them with one graph launch. */
@@ -376,6 +382,20 @@ int hgm_server_tuning(hgm_server *s, int qubits, hgm_tuning *out);
 int hgm_server_set_tuning(hgm_server *s, int qubits, const hgm_tuning *t);
 /* Milliseconds since the server was created (the clock of hgm_result). */
 double hgm_server_clock_ms(const hgm_server *s);
+/* Compiles the serving pass kernels of a circuit shape: `ops` (applied
+ * `iterations` times) is planned as a request of `qubits` qubits would be,
+ * and every pass structure without a kernel gets one (hgm_plan_serving_source;
+ * hipRTC, up to 16 compilations at once). From then on, requests of that
+ * qubit count are planned with HGM_PLAN_GREEDY | HGM_PLAN_FUSE (no row
+ * scaling, whose pivot choice depends on the matrix entries), so a circuit of
+ * the same shape with other matrices or control values (another seed, another
+ * marked value) finds its passes' kernels; its passes run them, and passes of
+ * other structures run the batched interpreter. States of one tile (whose
+ * whole circuit is one pass) always run the interpreter. *n_new: kernels
+ * added. No counterpart in the reference (the serving engine is this
+ * library's own). */
+int hgm_server_precompile(hgm_server *s, int qubits, const hgm_op *ops, int64_t n_ops, int iterations,
+                          int *n_new);
 /* Finishes every submitted request, then frees the server. */
 void hgm_server_destroy(hgm_server *s);
 

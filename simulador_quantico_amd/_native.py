@@ -124,6 +124,7 @@ SIGNATURES = [
                                  C.POINTER(C.c_int32)]),
     ("hgm_state_run_plan", C.c_int, [_P, _P, C.POINTER(Stats)]),
     ("hgm_plan_kernel", C.c_int, [_P]),
+    ("hgm_plan_serving_source", C.c_int, [_P, C.c_int64, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     ("hgm_plan_program", C.c_int, [_P, C.c_int64, _P, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int32), _P]),
     ("hgm_plan_maps", C.c_int, [_P, _P, C.POINTER(C.c_int64)]),
@@ -183,6 +184,7 @@ SIGNATURES = [
     ("hgm_server_tuning", C.c_int, [_P, C.c_int, C.POINTER(Tuning)]),
     ("hgm_server_set_tuning", C.c_int, [_P, C.c_int, C.POINTER(Tuning)]),
     ("hgm_server_clock_ms", C.c_double, [_P]),
+    ("hgm_server_precompile", C.c_int, [_P, C.c_int, C.POINTER(Op), C.c_int64, C.c_int, C.POINTER(C.c_int)]),
     ("hgm_server_destroy", None, [_P]),
     ("setDevice", C.c_bool, [C.c_int]),
     ("GpuExecutionWrapper", _P, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),

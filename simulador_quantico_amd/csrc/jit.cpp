@@ -492,6 +492,7 @@ void jit_generate(const Plan &plan, JitCode *out) {
 }
 
 int jit_compile(JitCode *out, std::string *err) {
+    if (!out->hash) out->hash = fnv1a(out->source);
     {
         std::lock_guard<std::mutex> lk(g_cache_mu);
         auto it = g_cache.find(out->hash);
@@ -549,6 +550,233 @@ int jit_compile(JitCode *out, std::string *err) {
 size_t jit_compiled_count() {
     std::lock_guard<std::mutex> lk(g_cache_mu);
     return g_cache.size();
+}
+
+// ---- serving pass kernels ---------------------------------------------------
+
+std::string serving_pass_key(const Plan &plan, const Pass &ps) {
+    const PassMem &pm = plan.mems[ps.mem];
+    std::string k;
+    auto put = [&](const void *p, size_t n) { k.append(static_cast<const char *>(p), n); };
+    const int32_t hdr[6] = {plan.qubits, plan.dtype, plan.shape.r, plan.shape.B, plan.shape.R, plan.shape.c};
+    put(hdr, sizeof hdr);
+    put(&ps.local_mask, sizeof(ps.local_mask));
+    put(&ps.phases, sizeof(ps.phases));
+    PassMem m = pm;
+    m.first_map = 0;
+    m.last_map = pm.last_map - pm.first_map;
+    put(&m, sizeof(m));
+    for (int32_t mi = pm.first_map; mi <= pm.last_map; ++mi) put(&plan.maps[mi], sizeof(PhaseMap));
+    for (int64_t i = 0; i < ps.rop_count; ++i) {
+        RegOp o = plan.rops[ps.rop_begin + i];
+        std::memset(o.m, 0, sizeof(o.m));
+        o.rval = 0;
+        o.gval = 0;
+        if (o.kind != R_DENSE_PIV && o.kind != R_DENSE_PIV_REAL) o.tval = 0;   // the pivot columns are code
+        if (o.kind == R_REMAP) o.rbit -= pm.first_map;
+        put(&o, sizeof(o));
+    }
+    return k;
+}
+
+namespace {
+
+// Code of one op on the register pair (k0, k1) of a serving kernel; the
+// coefficients are the op's locals m0..m7.
+std::string serving_pair(const Gen &g, const RegOp &op, int k0, int k1) {
+    const std::string a0 = reg(k0), a1 = reg(k1);
+    std::string o = "        { const " + g.V + " x = " + a0 + ", y = " + a1 + ";\n";
+    switch (op.kind) {
+        case R_DENSE:
+            o += "          " + a0 + ".x = m0 * x.x - m1 * x.y + m2 * y.x - m3 * y.y;\n";
+            o += "          " + a0 + ".y = m0 * x.y + m1 * x.x + m2 * y.y + m3 * y.x;\n";
+            o += "          " + a1 + ".x = m4 * x.x - m5 * x.y + m6 * y.x - m7 * y.y;\n";
+            o += "          " + a1 + ".y = m4 * x.y + m5 * x.x + m6 * y.y + m7 * y.x; }\n";
+            return o;
+        case R_DENSE_REAL:
+            o += "          " + a0 + ".x = m0 * x.x + m2 * y.x; " + a0 + ".y = m0 * x.y + m2 * y.y;\n";
+            o += "          " + a1 + ".x = m4 * x.x + m6 * y.x; " + a1 + ".y = m4 * x.y + m6 * y.y; }\n";
+            return o;
+        case R_ANTI:
+            o += "          " + a0 + ".x = m2 * y.x - m3 * y.y; " + a0 + ".y = m2 * y.y + m3 * y.x;\n";
+            o += "          " + a1 + ".x = m4 * x.x - m5 * x.y; " + a1 + ".y = m4 * x.y + m5 * x.x; }\n";
+            return o;
+        case R_SWAP:
+            o += "          " + a0 + " = y; " + a1 + " = x; }\n";
+            return o;
+        default: {   // R_DENSE_PIV, R_DENSE_PIV_REAL: out_i = v[p_i] + t_i * v[1 - p_i]
+            const char *v[2] = {"x", "y"};
+            const int p0 = (int)(op.tval & 1u), p1 = (int)(op.tval >> 1);
+            const std::string u0 = v[p0], w0 = v[1 - p0], u1 = v[p1], w1 = v[1 - p1];
+            if (op.kind == R_DENSE_PIV_REAL) {
+                o += "          " + a0 + ".x = " + u0 + ".x + m0 * " + w0 + ".x; " + a0 + ".y = " + u0 + ".y + m0 * " + w0 +
+                     ".y;\n";
+                o += "          " + a1 + ".x = " + u1 + ".x + m2 * " + w1 + ".x; " + a1 + ".y = " + u1 + ".y + m2 * " + w1 +
+                     ".y; }\n";
+            } else {
+                o += "          " + a0 + ".x = " + u0 + ".x + m0 * " + w0 + ".x - m1 * " + w0 + ".y;\n";
+                o += "          " + a0 + ".y = " + u0 + ".y + m0 * " + w0 + ".y + m1 * " + w0 + ".x;\n";
+                o += "          " + a1 + ".x = " + u1 + ".x + m2 * " + w1 + ".x - m3 * " + w1 + ".y;\n";
+                o += "          " + a1 + ".y = " + u1 + ".y + m2 * " + w1 + ".y + m3 * " + w1 + ".x; }\n";
+            }
+            return o;
+        }
+    }
+}
+
+}  // namespace
+
+std::string serving_kernel_source(const Plan &plan, const Pass &ps, const std::string &name, size_t *lds_bytes) {
+    Gen g;
+    g.f32 = plan.dtype == HGM_C64;
+    g.V = g.f32 ? "hgm_f2" : "hgm_d2";
+    g.T = g.f32 ? "float" : "double";
+    const uint64_t last = (uint64_t(1) << plan.shape.r) - 1;
+    const size_t entries = (size_t)padded((uint32_t)last) + 1;
+    g.split = entries * amp_bytes(plan.dtype) > 32 * 1024;
+    *lds_bytes = ps.phases > 1 ? entries * amp_bytes(plan.dtype) / (g.split ? 2 : 1) : 0;
+    const int B = plan.shape.B, A = 2 * plan.shape.R;
+    int tb = 0;
+    while ((1 << tb) < B) ++tb;
+    const PassMem &pm = plan.mems[ps.mem];
+    const uint64_t all = (plan.qubits >= 64) ? ~0ull : ((1ull << plan.qubits) - 1);
+    const uint64_t nonlocal = all & ~ps.local_mask;
+    uint64_t lt[kMaxThreadBits] = {0}, st[kMaxThreadBits] = {0};
+    for (int j = 0; j < tb; ++j) {
+        lt[j] = pm.lthr[j / 5][1 << (j % 5)];
+        st[j] = pm.sthr[j / 5][1 << (j % 5)];
+    }
+    std::string o;
+    o += "// generated serving pass kernel\n"
+         "struct __attribute__((aligned(16))) hgm_d2 { double x, y; };\n"
+         "struct __attribute__((aligned(8))) hgm_f2 { float x, y; };\n"
+         "struct hgm_item { unsigned long long state_off, ops_off, maps_off, pm_off, nonlocal_mask;\n"
+         "                  long long tile_begin, partials_off; int n_ops, pad; };\n"
+         "struct hgm_rop { int kind, rbit; unsigned rmask, rval, tmask, tval; unsigned long long gmask, gval;\n"
+         "                 double m[8]; };\n"
+         "struct hgm_part { double sum, pmax; unsigned long long idx, pad; };\n"
+         "static_assert(sizeof(hgm_item) == 64 && sizeof(hgm_rop) == 104 && sizeof(hgm_part) == 32, \"layouts\");\n";
+    o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(B) + ") " + name +
+         "(const hgm_item *__restrict__ items, int n_items, const char *__restrict__ progs,\n"
+         "        char *__restrict__ states, char *__restrict__ parts) {\n";
+    if (ps.phases > 1) o += "    extern __shared__ " + (g.split ? g.T : g.V) + " lds[];\n";
+    o += "    __shared__ double red_s[16], red_m[16];\n"
+         "    __shared__ unsigned long long red_i[16];\n"
+         "    const unsigned t = threadIdx.x;\n"
+         "    const long long b = blockIdx.x;\n"
+         "    int lo = 0, hi = n_items - 1;\n"
+         "    while (lo < hi) {\n"
+         "        const int mid = (lo + hi + 1) >> 1;\n"
+         "        if (items[mid].tile_begin <= b) lo = mid; else hi = mid - 1;\n"
+         "    }\n"
+         "    const hgm_item &it = items[lo];\n";
+    o += "    " + g.V + " *__restrict__ s = (" + g.V + " *)(states + it.state_off);\n";
+    o += "    const hgm_rop *__restrict__ ops = (const hgm_rop *)(progs + it.ops_off);\n"
+         "    const unsigned long long tid = (unsigned long long)(b - it.tile_begin);\n";
+    o += "    const unsigned long long base = " + base_expr(nonlocal) + ";\n";
+    o += "    const unsigned long long lt = " + thread_or(lt, tb) + ";\n";
+    o += "    const unsigned long long st = " + thread_or(st, tb) + ";\n";
+    const int m_first = pm.first_map, m_last = pm.last_map;
+    if (ps.phases > 1)
+        for (int mi = m_first; mi <= m_last; ++mi) {
+            uint32_t c[kMaxThreadBits] = {0};
+            for (int j = 0; j < tb; ++j) c[j] = padded(unswizzle(plan.maps[mi].thr[j / 5][1 << (j % 5)]));
+            o += "    const unsigned f" + std::to_string(mi - m_first) + " = " + thread_sum(c, tb) + ";\n";
+        }
+    for (int k = 0; k < A; ++k)
+        o += "    " + g.V + " " + reg(k) + " = s[(base | " + hex64(pm.lreg[k]) + ") + lt];\n";
+    int cur = m_first;
+    for (int64_t i = 0; i < ps.rop_count; ++i) {
+        const RegOp &op = plan.rops[ps.rop_begin + i];
+        const std::string oi = "ops[" + std::to_string(i) + "]";
+        if (op.kind == R_REMAP) {
+            const int nxt = op.rbit;
+            const char *parts[2][2] = {{"", nullptr}, {".x", ".y"}};
+            for (int h = 0; h < 2 && parts[g.split][h]; ++h) {
+                const std::string part = parts[g.split][h];
+                o += "    __syncthreads();\n";
+                for (int k = 0; k < A; ++k)
+                    o += "    lds[f" + std::to_string(cur - m_first) + " + " +
+                         std::to_string(padded(unswizzle(plan.maps[cur].reg[k]))) + "u] = " + reg(k) + part + ";\n";
+                o += "    __syncthreads();\n";
+                for (int k = 0; k < A; ++k)
+                    o += "    " + reg(k) + part + " = lds[f" + std::to_string(nxt - m_first) + " + " +
+                         std::to_string(padded(unswizzle(plan.maps[nxt].reg[k]))) + "u];\n";
+            }
+            cur = nxt;
+            continue;
+        }
+        // the op's coefficients and control values are read from the item's
+        // register program; its kind, register bit, masks and pivots are code
+        std::string body = "    {\n";
+        int nm = 0;
+        switch (op.kind) {
+            case R_DENSE: nm = 8; break;
+            case R_DENSE_REAL: nm = 8; break;
+            case R_ANTI: nm = 6; break;
+            case R_PHASE: nm = 2; break;
+            case R_PHASE_REAL: nm = 1; break;
+            case R_DENSE_PIV: nm = 4; break;
+            case R_DENSE_PIV_REAL: nm = 3; break;
+            default: nm = 0;
+        }
+        for (int e = 0; e < nm; ++e)
+            body += "        const " + g.T + " m" + std::to_string(e) + " = (" + g.T + ")" + oi + ".m[" + std::to_string(e) +
+                    "];\n";
+        if (op.rmask) body += "        const unsigned rv = " + oi + ".rval;\n";
+        const bool phase = op.kind == R_PHASE || op.kind == R_PHASE_REAL;
+        for (int k0 = 0; k0 < A; ++k0) {
+            std::string one;
+            if (phase) {
+                const std::string a = reg(k0);
+                if (op.kind == R_PHASE_REAL) one = "        " + a + ".x *= m0; " + a + ".y *= m0;\n";
+                else
+                    one = "        { const " + g.V + " x = " + a + "; " + a + ".x = m0 * x.x - m1 * x.y; " + a +
+                          ".y = m0 * x.y + m1 * x.x; }\n";
+            } else {
+                if ((k0 >> op.rbit) & 1) continue;
+                one = serving_pair(g, op, k0, k0 | (1 << op.rbit));
+            }
+            if (op.rmask) body += "        if ((" + std::to_string(k0) + "u & " + hex32(op.rmask) + ") == rv) {\n" + one + "        }\n";
+            else body += one;
+        }
+        std::string cond;
+        if (op.gmask) cond += "(base & " + hex64(op.gmask) + ") == " + oi + ".gval";
+        if (op.tmask) cond += std::string(cond.empty() ? "" : " && ") + "(t & " + hex32(op.tmask) + ") == " + oi + ".tval";
+        if (!cond.empty()) body = "    if (" + cond + ")\n" + body;
+        o += body + "    }\n";
+    }
+    for (int k = 0; k < A; ++k)
+        o += "    s[(base | " + hex64(pm.sreg[k]) + ") + st] = " + reg(k) + ";\n";
+    // the last pass of a state reduces its readout per tile: FP64 sum of |a|^2
+    // and the largest |a|^2 with the lowest index
+    const int width = B < 64 ? B : 64, waves = (B + 63) / 64;
+    o += "    if (it.partials_off < 0) return;\n"
+         "    double sum = 0.0, best = -1.0;\n"
+         "    unsigned long long bi = ~0ull;\n";
+    for (int k = 0; k < A; ++k)
+        o += "    { const double p = (double)" + reg(k) + ".x * (double)" + reg(k) + ".x + (double)" + reg(k) +
+             ".y * (double)" + reg(k) + ".y;\n      const unsigned long long idx = (base | " + hex64(pm.sreg[k]) +
+             ") | st;\n      sum += p;\n      if (p > best || (p == best && idx < bi)) { best = p; bi = idx; } }\n";
+    o += "    for (int off = " + std::to_string(width >> 1) + "; off > 0; off >>= 1) {\n"
+         "        sum += __shfl_xor(sum, off, " + std::to_string(width) + ");\n"
+         "        const double ob = __shfl_xor(best, off, " + std::to_string(width) + ");\n"
+         "        const unsigned long long oi = __shfl_xor(bi, off, " + std::to_string(width) + ");\n"
+         "        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }\n"
+         "    }\n"
+         "    if ((t & 63u) == 0) { red_s[t >> 6] = sum; red_m[t >> 6] = best; red_i[t >> 6] = bi; }\n"
+         "    __syncthreads();\n"
+         "    if (t == 0) {\n"
+         "        for (int w = 1; w < " + std::to_string(waves) + "; ++w) {\n"
+         "            sum += red_s[w];\n"
+         "            if (red_m[w] > best || (red_m[w] == best && red_i[w] < bi)) { best = red_m[w]; bi = red_i[w]; }\n"
+         "        }\n"
+         "        hgm_part r;\n"
+         "        r.sum = sum; r.pmax = best; r.idx = bi; r.pad = 0;\n"
+         "        ((hgm_part *)(parts + it.partials_off))[tid] = r;\n"
+         "    }\n"
+         "}\n";
+    return o;
 }
 
 }  // namespace hgm

@@ -37,4 +37,15 @@ int jit_compile(JitCode *out, std::string *err);
 // Number of code objects in the compile cache.
 size_t jit_compiled_count();
 
+// Serving pass kernels. A register-kernel pass's kernel for the serving engine
+// is a function of its structure only: op kinds, register bits, condition
+// masks, pivot columns, phase maps and load/store tables, the qubit count and
+// tile shape. Matrix entries and control values are read at run time from
+// the item's register program (as the interpreter reads them), so requests
+// whose passes have equal keys share one kernel. The kernel takes the batched
+// interpreter's arguments (items, n_items, programs, states, partials) and
+// also reduces the readout of an item that asks for it.
+std::string serving_pass_key(const Plan &plan, const Pass &ps);
+std::string serving_kernel_source(const Plan &plan, const Pass &ps, const std::string &name, size_t *lds_bytes);
+
 }  // namespace hgm

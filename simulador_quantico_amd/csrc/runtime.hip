@@ -1089,6 +1089,19 @@ int hgm_plan_jit_source(const hgm_plan *p, char *out, size_t cap, size_t *len) {
     return HGM_OK;
 }
 
+int hgm_plan_serving_source(const hgm_plan *p, int64_t pass, char *out, size_t cap, size_t *len) {
+    if (!p || !len || pass < 0 || pass >= (int64_t)p->plan.passes.size()) { t_err = "bad arguments"; return HGM_E_ARG; }
+    if (p->plan.kernel != HGM_KERNEL_REG || p->plan.passes[pass].op_count == 0) {
+        t_err = "serving kernels are generated for the non-empty passes of register-kernel plans";
+        return HGM_E_ARG;
+    }
+    size_t lds = 0;
+    const std::string src = serving_kernel_source(p->plan, p->plan.passes[pass], "hgm_serve", &lds);
+    *len = src.size();
+    if (out && cap > src.size()) std::memcpy(out, src.c_str(), src.size() + 1);
+    return HGM_OK;
+}
+
 double hgm_plan_compile_seconds(const hgm_plan *p) { return (p && p->plan.jit) ? p->plan.jit->compile_s : 0.0; }
 
 int hgm_plan_program(const hgm_plan *p, int64_t i, void *rops, int64_t *n_rops, int32_t *first_map,

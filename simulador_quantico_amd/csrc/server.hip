@@ -24,10 +24,13 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "hgm_dgm.hpp"
 #include "hgm_gpu.h"
+#include "jit.hpp"
 #include "kernels.hpp"
 #include "planner.hpp"
 #include "runtime.hpp"
@@ -123,6 +126,7 @@ struct Req {
     std::string err;
     Plan plan;
     std::vector<int> passes;       // non-empty passes in order
+    std::vector<int> kernel;       // per entry of passes: compiled serving kernel, or -1 (interpreter)
     size_t next = 0;
     int lane = 0;
     int64_t n_tiles = 1;
@@ -157,6 +161,21 @@ double now_ms(std::chrono::steady_clock::time_point t0) {
 // and cheaper gates: Grover's X-H-X runs become one real gate per qubit, and
 // each dense update costs 8 multiply-adds per pair, 4 when real).
 constexpr int kServingPlanner = HGM_PLAN_GREEDY | HGM_PLAN_FUSE | HGM_PLAN_PIVOT;
+
+// Planner of the qubit counts that have compiled pass kernels: row scaling
+// picks each row's pivot by magnitude, which makes a pass's structure (and so
+// its kernel) depend on the matrix entries; without it, circuits of one shape
+// (a random-layer circuit under any seed, Grover for any marked value) share
+// their pass kernels.
+constexpr int kCompiledPlanner = HGM_PLAN_GREEDY | HGM_PLAN_FUSE;
+
+// A compiled serving pass kernel (serving_kernel_source), loaded on the
+// server's device.
+struct ServeKernel {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    size_t lds = 0;
+};
 
 // Tile shape the serving engine uses for a qubit count when no autotuning is
 // asked for: 16 amplitudes per thread; states up to 13 qubits are one tile;
@@ -247,6 +266,11 @@ struct hgm_server {
 
     std::thread worker;
     std::vector<std::thread> planners;
+
+    // compiled pass kernels (hgm_server_precompile), by serving_pass_key
+    std::mutex kmu;
+    std::unordered_map<std::string, int> kernel_of;
+    std::deque<ServeKernel> kernels;          // stable addresses while precompile appends
 };
 
 namespace {
@@ -305,6 +329,15 @@ void planner_loop(hgm_server *s) {
             for (size_t i = 0; i < r->plan.passes.size(); ++i)
                 if (r->plan.passes[i].op_count > 0) r->passes.push_back((int)i);
             if (r->passes.empty()) { rc = HGM_E_ARG; err = "the request applies no gate"; }
+        }
+        if (rc == HGM_OK) {   // the compiled kernel of each pass, where one exists
+            r->kernel.assign(r->passes.size(), -1);
+            std::lock_guard<std::mutex> g(s->kmu);
+            if (!s->kernel_of.empty() && r->plan.passes.size() > 1)
+                for (size_t i = 0; i < r->passes.size(); ++i) {
+                    auto it = s->kernel_of.find(serving_pass_key(r->plan, r->plan.passes[r->passes[i]]));
+                    if (it != s->kernel_of.end()) r->kernel[i] = it->second;
+                }
         }
         if (rc == HGM_OK) {
             const Plan &p = r->plan;
@@ -410,19 +443,24 @@ bool launch_step(hgm_server *s, Lane &L) {
     ReadoutPartial *d_results = reinterpret_cast<ReadoutPartial *>(d + s->results_off);
 
     std::vector<Req *> order(L.active.begin(), L.active.end());
-    std::stable_sort(order.begin(), order.end(), [](const Req *a, const Req *b) {
+    auto kern = [](const Req *r) { return r->kernel[r->next]; };
+    std::stable_sort(order.begin(), order.end(), [&](const Req *a, const Req *b) {
         if (a->plan.shape.R != b->plan.shape.R) return a->plan.shape.R < b->plan.shape.R;
-        return a->plan.shape.B < b->plan.shape.B;
+        if (a->plan.shape.B != b->plan.shape.B) return a->plan.shape.B < b->plan.shape.B;
+        return kern(a) < kern(b);
     });
-    struct Group { int A, B; int first, count; int64_t tiles; size_t lds; };
+    // one launch per workgroup shape and kernel: the interpreter (-1) or a
+    // compiled pass kernel shared by the items whose passes have its structure
+    struct Group { int A, B, kernel; int first, count; int64_t tiles; size_t lds; };
     std::vector<Group> groups;
     Step step;
     step.slot = slot;
     int n_tasks = 0;
     for (int i = 0; i < (int)order.size(); ++i) {
         Req *r = order[i];
-        const int A = 2 * r->plan.shape.R, B = r->plan.shape.B;
-        if (groups.empty() || groups.back().A != A || groups.back().B != B) groups.push_back({A, B, i, 0, 0, 0});
+        const int A = 2 * r->plan.shape.R, B = r->plan.shape.B, K = kern(r);
+        if (groups.empty() || groups.back().A != A || groups.back().B != B || groups.back().kernel != K)
+            groups.push_back({A, B, K, i, 0, 0, 0});
         Group &g = groups.back();
         const Pass &ps = r->plan.passes[r->passes[r->next]];
         const bool last = r->next + 1 == r->passes.size();
@@ -456,8 +494,22 @@ bool launch_step(hgm_server *s, Lane &L) {
                            L.stream);
     for (const Group &g : groups) {
         if (e != hipSuccess) break;
-        e = launch_batch_reg(s->cfg.dtype, g.A, g.B, reinterpret_cast<const BatchItem *>(d) + g.first, g.count,
-                             g.tiles, g.lds, s->arena.base(), L.stream);
+        const BatchItem *gi = reinterpret_cast<const BatchItem *>(d) + g.first;
+        if (g.kernel < 0) {
+            e = launch_batch_reg(s->cfg.dtype, g.A, g.B, gi, g.count, g.tiles, g.lds, s->arena.base(), L.stream);
+            continue;
+        }
+        ServeKernel k;
+        {
+            std::lock_guard<std::mutex> lk(s->kmu);
+            k = s->kernels[(size_t)g.kernel];
+        }
+        if ((uint64_t)g.tiles * (uint64_t)g.B > 0xffffffffull) { e = hipErrorInvalidValue; break; }
+        int count = g.count;
+        char *arena = s->arena.base();
+        void *args[] = {&gi, &count, &arena, &arena, &arena};
+        e = hipModuleLaunchKernel(k.fn, (unsigned)g.tiles, 1, 1, (unsigned)g.B, 1, 1, (unsigned)k.lds, L.stream,
+                                  args, nullptr);
     }
     if (e == hipSuccess && n_tasks)
         e = launch_readout_final(reinterpret_cast<const ReadoutTask *>(d + s->tasks_off), n_tasks, L.stream);
@@ -719,6 +771,84 @@ int hgm_server_set_tuning(hgm_server *s, int qubits, const hgm_tuning *t) {
     return HGM_OK;
 }
 
+int hgm_server_precompile(hgm_server *s, int qubits, const hgm_op *ops, int64_t n_ops, int iterations, int *n_new) {
+    if (n_new) *n_new = 0;
+    if (!s || !ops || n_ops < 1 || iterations < 1 || qubits < 1 || qubits > kMaxQubits) {
+        set_error("bad argument");
+        return HGM_E_ARG;
+    }
+    hgm_tuning t;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        t = s->tune[qubits];
+        if (qubits <= t.gpu_region_bits) return HGM_OK;   // one tile: the interpreter runs the whole circuit
+        t.planner = kCompiledPlanner;
+        s->tune[qubits] = t;   // requests planned from now on find the kernels
+    }
+    Plan plan;
+    std::string err;
+    int rc = build_plan(qubits, s->cfg.dtype, ops, n_ops, iterations, t, &plan, &err);
+    if (rc) { set_error(err); return rc; }
+    if (plan.kernel != HGM_KERNEL_REG) { set_error("serving needs the register kernel for this tile shape"); return HGM_E_TUNING; }
+    struct Job {
+        std::string key;
+        JitCode code;
+        size_t lds = 0;
+        int rc = HGM_OK;
+        std::string err;
+    };
+    std::vector<Job> jobs;
+    {
+        std::lock_guard<std::mutex> g(s->kmu);
+        std::unordered_set<std::string> seen;
+        for (const Pass &ps : plan.passes) {
+            if (!ps.op_count) continue;
+            std::string key = serving_pass_key(plan, ps);
+            if (s->kernel_of.count(key) || !seen.insert(key).second) continue;
+            Job j;
+            j.key = std::move(key);
+            j.code.source = serving_kernel_source(plan, ps, "hgm_serve", &j.lds);
+            j.code.names = {"hgm_serve"};
+            jobs.push_back(std::move(j));
+        }
+    }
+    // hipRTC compiles independent programs concurrently
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nth = std::min<size_t>(jobs.size(), std::min(16u, hw));
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (size_t w = 0; w < nth; ++w)
+        pool.emplace_back([&] {
+            for (size_t i; (i = next.fetch_add(1)) < jobs.size();) jobs[i].rc = jit_compile(&jobs[i].code, &jobs[i].err);
+        });
+    for (std::thread &th : pool) th.join();
+    int prev = 0;
+    hipGetDevice(&prev);
+    hipSetDevice(s->device);
+    int added = 0;
+    for (Job &j : jobs) {
+        if (j.rc) { rc = j.rc; set_error(j.err); break; }
+        if (j.lds > 64 * 1024) continue;   // above the default dynamic LDS limit: stays interpreted
+        ServeKernel k;
+        k.lds = j.lds;
+        hipError_t e = hipModuleLoadData(&k.mod, j.code.binary.data());
+        if (e == hipSuccess) e = hipModuleGetFunction(&k.fn, k.mod, "hgm_serve");
+        if (e != hipSuccess) {
+            if (k.mod) hipModuleUnload(k.mod);
+            set_error(hip_msg(e, "loading a serving pass kernel"));
+            rc = HGM_E_HIP;
+            break;
+        }
+        std::lock_guard<std::mutex> g(s->kmu);
+        s->kernel_of[j.key] = (int)s->kernels.size();
+        s->kernels.push_back(k);
+        ++added;
+    }
+    hipSetDevice(prev);
+    if (n_new) *n_new = added;
+    return rc;
+}
+
 double hgm_server_clock_ms(const hgm_server *s) { return s ? now_ms(s->t0) : 0.0; }
 
 void hgm_server_destroy(hgm_server *s) {
@@ -734,6 +864,8 @@ void hgm_server_destroy(hgm_server *s) {
     release_lanes(s);
     s->arena.destroy();
     s->staging.destroy();
+    for (ServeKernel &k : s->kernels)
+        if (k.mod) hipModuleUnload(k.mod);
     delete s;
 }
 

@@ -83,6 +83,24 @@ def build_job(index: int, arrival_s: float, kind: str, n: int, rng: np.random.Ge
     raise KeyError(kind)
 
 
+def mix_shapes(n: int, grover_iterations=(1, 2, 3, 4), random_depth: int = 10):
+    """One circuit of each shape the mix draws at `n` qubits, for
+    Server.precompile: the Hadamard layer, Grover with 1-4 iterations (their
+    passes already hold every pass structure of 1-64 iterations), the Draper
+    adder and random layers. Matrix entries and control values do not matter:
+    kernels are shared by structure."""
+    h = _circuit_ops([Hadamard(n, 0, n)])
+    it = _circuit_ops(GroverIteration(n, (1 << n) - 1))
+    out = [(h, 1)] + [(np.concatenate([h] + [it] * k), 1) for k in grover_iterations]
+    w = (n - 1) // 2
+    if w >= 1:
+        d = DGM()
+        out.append((_circuit_ops(DraperAdder(w, d), d), 1))
+    d = DGM()
+    out.append((_circuit_ops(RandomLayers(n, random_depth, 1, d), d), 1))
+    return out
+
+
 def make_mix(n_requests: int = 4096, rate: float = 1000.0, seed: int = 2026, qmin: int = 10, qmax: int = 26,
              grover_max_iter: int = 64, random_depth: int = 10) -> list[Job]:
     """The serving mix: Poisson arrivals at `rate` requests/s (first at t=0)."""
@@ -161,6 +179,14 @@ class Server:
         for k, v in tuning.items():
             setattr(t, k, int(v))
         check(lib.hgm_server_set_tuning(self._h, qubits, C.byref(t)))
+
+    def precompile(self, qubits: int, ops: np.ndarray, iterations: int = 1) -> int:
+        """Compiles the pass kernels of this circuit shape (hgm_server_precompile);
+        returns the number of kernels added."""
+        ops = np.ascontiguousarray(ops, dtype=OP_NP)
+        n = C.c_int()
+        check(lib.hgm_server_precompile(self._h, qubits, _op_ptr(ops), len(ops), iterations, C.byref(n)))
+        return n.value
 
     def clock_ms(self) -> float:
         return float(lib.hgm_server_clock_ms(self._h))

@@ -433,3 +433,45 @@ def test_measure_many_matches_reference():
         k = r.lib.ref_measure_many(r.h, q.ctypes.data, len(qs), keys.ctypes.data, vals.ctypes.data, 1 << n)
         theirs = {int(keys[i]): float(vals[i]) for i in range(k)}
         assert ours == theirs, qs
+
+def _serving_sources(n, ops, dtype=np.complex64):
+    arr = (N.Op * len(ops)).from_buffer_copy(ops.tobytes())
+    t = N.default_tuning(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4,
+                         kernel=N.HGM_KERNEL_REG, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE, ops_block=1 << 30)
+    p = hq.Plan(n, arr, t, 1, dtype)
+    out = []
+    for i in range(len(p)):
+        ln = C.c_size_t()
+        if N.lib.hgm_plan_serving_source(p._h, i, None, 0, C.byref(ln)) != 0:
+            continue
+        b = C.create_string_buffer(ln.value + 1)
+        N.check(N.lib.hgm_plan_serving_source(p._h, i, b, ln.value + 1, C.byref(ln)))
+        out.append(b.value.decode())
+    return out
+
+
+def test_serving_kernels_depend_on_the_circuit_shape_only():
+    """The serving engine's compiled pass kernels hold no matrix entry or
+    control value: circuits of one shape (another seed, another marked value)
+    have the same kernel sources, other shapes and dtypes do not, and the
+    values are read from the register program (ops[i].m, .rval, .gval)."""
+    from simulador_quantico_amd import serving as S
+    n = 16
+    g = [np.concatenate([S._circuit_ops([hq.Hadamard(n, 0, n)])] +
+                        [S._circuit_ops(hq.GroverIteration(n, v))] * 3) for v in (5, 40000)]
+    r = []
+    for seed in (1, 2):
+        d = hq.DGM()
+        r.append(S._circuit_ops(hq.RandomLayers(n, 10, seed, d), d))
+    assert not np.array_equal(g[0]["m"], g[1]["m"]) or not np.array_equal(g[0]["control_value"], g[1]["control_value"])
+    assert not np.array_equal(r[0]["m"], r[1]["m"])
+    sg = [_serving_sources(n, x) for x in g]
+    sr = [_serving_sources(n, x) for x in r]
+    assert len(sg[0]) > 2 and sg[0] == sg[1]
+    assert len(sr[0]) > 1 and sr[0] == sr[1]
+    assert sr[0] != _serving_sources(n, r[0], np.complex128)
+    d = hq.DGM()
+    assert sr[0][0] != _serving_sources(n + 1, S._circuit_ops(hq.RandomLayers(n + 1, 10, 1, d), d))[0]
+    src = "".join(sr[0] + sg[0])
+    assert "ops[0].m[0]" in src and ".rval" in src and ".gval" in src
+    assert "0x1.6a09e6" not in src and "0x1.6a09e667f3bcdp-1" not in src     # no 1/sqrt(2) literal
