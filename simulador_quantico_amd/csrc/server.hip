@@ -333,7 +333,7 @@ void planner_loop(hgm_server *s) {
         if (rc == HGM_OK) {   // the compiled kernel of each pass, where one exists
             r->kernel.assign(r->passes.size(), -1);
             std::lock_guard<std::mutex> g(s->kmu);
-            if (!s->kernel_of.empty() && r->plan.passes.size() > 1)
+            if (!s->kernel_of.empty() && r->qubits > r->plan.shape.r)   // one-tile states stay interpreted
                 for (size_t i = 0; i < r->passes.size(); ++i) {
                     auto it = s->kernel_of.find(serving_pass_key(r->plan, r->plan.passes[r->passes[i]]));
                     if (it != s->kernel_of.end()) r->kernel[i] = it->second;
