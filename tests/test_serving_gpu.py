@@ -84,6 +84,10 @@ def test_compiled_pass_kernels_against_oracle_alone_and_in_mix(dtype):
         say("compiled serving kernels:", added, "again:", again, [(j.kind, j.qubits) for j in jobs])
         assert added > 0 and again == 0
         assert srv.precompile(12, jobs[0].ops) == 0     # one tile: the interpreter runs it whole
+        with pytest.raises(N.HGMError):
+            srv.precompile(15, jobs[0].ops[:0])           # no gate
+        with pytest.raises(N.HGMError):
+            srv.precompile(41, jobs[0].ops)               # qubit count out of range
         alone, ra = run_collect(srv, jobs, dtype, together=False)
         mixed, rm = run_collect(srv, jobs, dtype, together=True)
     assert {j.qubits for j in jobs} >= {12, 14, 15, 16}
