@@ -107,6 +107,29 @@ def test_compiled_pass_kernels_against_oracle_alone_and_in_mix(dtype):
             assert rm[j.index]["argmax"] == j.expect
 
 
+def test_precompile_while_requests_are_in_flight():
+    """Kernels added while requests of the same qubit counts are queued,
+    planned and running: requests planned before run the interpreter, later
+    ones the new kernels, and every result is right."""
+    jobs = small_mix(24, seed=33, qmin=14, qmax=16)
+    with S.Server(np.complex64, max_active=8, hbm_budget_gib=1) as srv:
+        outs = {j.index: np.zeros(1 << j.qubits, np.complex64) for j in jobs}
+        ids = {}
+        for k, j in enumerate(jobs):
+            ids[srv.submit(j.qubits, j.ops, j.basis, state_out=outs[j.index])] = j
+            if k == 6:
+                for n in (14, 15, 16):
+                    for ops, iters in S.mix_shapes(n):
+                        srv.precompile(n, ops, iters)
+        srv.wait_all()
+        res = {r["id"]: r for r in srv.poll(cap=64)}
+    assert len(res) == len(jobs)
+    for rid, j in ids.items():
+        assert res[rid]["status"] == 0
+        f = S.fidelity(oracle_run_job(j), outs[j.index])
+        assert f >= 1 - 1e-6, (j.kind, j.qubits, f)
+
+
 def test_served_fp64_matches_oracle():
     jobs = small_mix(10, seed=8, qmin=10, qmax=15)
     with S.Server(np.complex128, max_active=16, hbm_budget_gib=1) as srv:
