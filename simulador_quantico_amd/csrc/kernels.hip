@@ -386,7 +386,7 @@ __device__ __forceinline__ void reg_remap(typename Vec<T>::type (&a)[A], typenam
 }
 
 template <class T, int A>
-__global__ void __launch_bounds__(reg_kernel_max_threads(A)) k_pass_reg(typename Vec<T>::type *__restrict__ state,
+__global__ void __launch_bounds__(reg_kernel_max_threads(A, sizeof(typename Vec<T>::type))) k_pass_reg(typename Vec<T>::type *__restrict__ state,
                                                    const RegOp *__restrict__ ops, int n_ops,
                                                    const PhaseMap *__restrict__ maps, const PassMem *__restrict__ pm,
                                                    uint64_t nonlocal_mask, int64_t n_tiles, int r) {
@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(reg_kernel_max_threads(A)) k_pass_reg(typename
 // sum of |a|^2, largest |a|^2 with the lowest index on ties) into
 // partials[tile]; k_readout_final combines the tiles of a state.
 template <class T, int A>
-__global__ void __launch_bounds__(reg_kernel_max_threads(A)) k_batch_reg(const BatchItem *__restrict__ items,
+__global__ void __launch_bounds__(reg_kernel_max_threads(A, sizeof(typename Vec<T>::type))) k_batch_reg(const BatchItem *__restrict__ items,
                                                                           int n_items,
                                                                           const char *__restrict__ progs,
                                                                           char *__restrict__ states,
@@ -811,7 +811,7 @@ hipError_t launch_collapse(int dtype, void *state, uint64_t n, int bit, int outc
 
 hipError_t launch_batch_reg(int dtype, int A, int B, const BatchItem *items, int n_items, int64_t total_tiles,
                             size_t lds_bytes, void *arena, hipStream_t s) {
-    if (n_items < 1 || total_tiles < 1 || B < 1 || (uint64_t)total_tiles > max_blocks(B) || B > reg_kernel_max_threads(A) ||
+    if (n_items < 1 || total_tiles < 1 || B < 1 || (uint64_t)total_tiles > max_blocks(B) || B > reg_kernel_max_threads(A, amp_bytes(dtype)) ||
         !arena)
         return hipErrorInvalidValue;
     char *base = static_cast<char *>(arena);

@@ -860,10 +860,10 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
         }
     }
     const int A = 2 * plan.shape.R;
-    const bool reg_ok = A <= kMaxRegAmps && plan.shape.B <= reg_kernel_max_threads(A);
+    const bool reg_ok = A <= kMaxRegAmps && plan.shape.B <= reg_kernel_max_threads(A, amp_bytes(dtype));
     if ((t.kernel == HGM_KERNEL_REG || t.kernel == HGM_KERNEL_JIT) && !reg_ok) {
         *err = "Erro de tuning: the register kernel holds at most " + std::to_string(kMaxRegAmps) +
-               " amplitudes per thread and at most " + std::to_string(reg_kernel_max_threads(A)) +
+               " amplitudes per thread and at most " + std::to_string(reg_kernel_max_threads(A, amp_bytes(dtype))) +
                " threads for 2*repeat_count = " + std::to_string(A);
         return HGM_E_TUNING;
     }

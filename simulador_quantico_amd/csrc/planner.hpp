@@ -52,10 +52,13 @@ constexpr int kMaxTileBits = 16;
 constexpr int kMaxRegAmps = 32;      // amplitudes per thread of the register kernel
 constexpr int kMaxThreadBits = 10;   // 1024 threads per workgroup
 
-// Workgroup size limit of the register kernel for A amplitudes per thread:
-// the VGPR budget per lane is 512 / (waves per SIMD), and A complex doubles
-// take 4A VGPRs.
-constexpr int reg_kernel_max_threads(int A) { return A >= 32 ? 256 : (A >= 16 ? 512 : 1024); }
+// Workgroup size limit of the register kernel for A amplitudes per thread of
+// amp_bytes bytes: the VGPR budget per lane is 512 / (waves per SIMD); A
+// complex doubles take 4A VGPRs, A complex floats 2A (FP32 with 16 amplitudes
+// per thread fits 128 VGPRs, so 1024 threads can hold a 2^14-amplitude tile).
+constexpr int reg_kernel_max_threads(int A, size_t amp_bytes) {
+    return amp_bytes <= 8 ? (A >= 32 ? 512 : 1024) : (A >= 32 ? 256 : (A >= 16 ? 512 : 1024));
+}
 
 struct TileShape {
     int r;      // tile bits

@@ -169,6 +169,10 @@ constexpr int kServingPlanner = HGM_PLAN_GREEDY | HGM_PLAN_FUSE | HGM_PLAN_PIVOT
 // their pass kernels.
 constexpr int kCompiledPlanner = HGM_PLAN_GREEDY | HGM_PLAN_FUSE;
 
+// Dynamic LDS a compiled serving kernel may ask for: the CU's 160 KiB less
+// the kernel's static readout arrays.
+constexpr size_t kServeMaxLds = 160 * 1024 - 1024;
+
 // A compiled serving pass kernel (serving_kernel_source), loaded on the
 // server's device.
 struct ServeKernel {
@@ -184,12 +188,12 @@ bool serving_amps_ok(int A) { return A == 2 || A == 4 || A == 8 || A == 16 || A 
 // asked for: 16 amplitudes per thread; states up to 13 qubits are one tile;
 // larger ones use tiles of 2^12 amplitudes. The always-local low bits make
 // 128-byte runs: 3 bits of 16-byte FP64 amplitudes, 4 bits of 8-byte FP32 ones.
-// FP32 states of 21-23 qubits use tiles of 2^13 (with 3 always-local bits at
-// 23 qubits, 64-byte runs): their n - c non-coalesced bits then fit two
-// passes' windows of r - c bits, so a layer of gates on every qubit takes 2
-// passes instead of 3-4 (Grover with compiled pass kernels: 28-34% faster at
-// 21-23 qubits; from 24 qubits on no 2^13 tile with 64-byte runs or longer
-// fits two windows).
+// FP32 states of 21-22 qubits use tiles of 2^13 (512 threads) and of 23-24
+// qubits tiles of 2^14 (1024 threads): their n - 4 non-coalesced bits then
+// fit two passes' windows of r - 4 bits, so a layer of gates on every qubit
+// takes 2 passes instead of 3-4 (Grover with compiled pass kernels: 28-31%
+// faster at 21-24 qubits; at 25-26 qubits no tile the register kernels hold
+// fits two windows with 128-byte runs).
 hgm_tuning serving_default_tuning(int n, int dtype) {
     const int c = dtype == HGM_C64 ? 4 : 3;
     hgm_tuning t;
@@ -209,11 +213,10 @@ hgm_tuning serving_default_tuning(int n, int dtype) {
         t.gpu_region_bits = 12;
         t.gpu_coalesced_bits = c;
         if (dtype == HGM_C64 && n - c > 2 * (12 - c))
-            for (int cc = c; cc >= 3; --cc)
-                if (n - cc <= 2 * (13 - cc)) {
-                    t.block_size = 512;
-                    t.gpu_region_bits = 13;
-                    t.gpu_coalesced_bits = cc;
+            for (int r : {13, 14})
+                if (n - c <= 2 * (r - c)) {
+                    t.block_size = int(1) << (r - 4);
+                    t.gpu_region_bits = r;
                     break;
                 }
     }
@@ -779,7 +782,7 @@ int hgm_server_set_tuning(hgm_server *s, int qubits, const hgm_tuning *t) {
     if (int rc = make_shape(qubits, s->cfg.dtype, *t, &shape, &err)) { set_error(err); return rc; }
     const int A = 2 * shape.R;
     if (!serving_amps_ok(A)) { set_error("the serving kernel holds 2, 4, 8, 16 or 32 amplitudes per thread"); return HGM_E_TUNING; }
-    if (shape.B > reg_kernel_max_threads(A)) { set_error("block_size above the register kernel's limit for this repeat_count"); return HGM_E_TUNING; }
+    if (shape.B > reg_kernel_max_threads(A, amp_bytes(s->cfg.dtype))) { set_error("block_size above the register kernel's limit for this repeat_count"); return HGM_E_TUNING; }
     std::lock_guard<std::mutex> g(s->mu);   // planner threads copy their entry under this lock
     s->tune[qubits] = *t;
     return HGM_OK;
@@ -842,11 +845,14 @@ int hgm_server_precompile(hgm_server *s, int qubits, const hgm_op *ops, int64_t 
     int added = 0;
     for (Job &j : jobs) {
         if (j.rc) { rc = j.rc; set_error(j.err); break; }
-        if (j.lds > 64 * 1024) continue;   // above the default dynamic LDS limit: stays interpreted
+        if (j.lds > kServeMaxLds) continue;   // stays interpreted
         ServeKernel k;
         k.lds = j.lds;
         hipError_t e = hipModuleLoadData(&k.mod, j.code.binary.data());
         if (e == hipSuccess) e = hipModuleGetFunction(&k.fn, k.mod, "hgm_serve");
+        if (e == hipSuccess && j.lds > 64 * 1024)   // above the default dynamic LDS of a launch
+            e = hipFuncSetAttribute(reinterpret_cast<const void *>(k.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)j.lds);
         if (e != hipSuccess) {
             if (k.mod) hipModuleUnload(k.mod);
             set_error(hip_msg(e, "loading a serving pass kernel"));
@@ -897,7 +903,7 @@ int hgm_autotune(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int it
     for (int A : {8, 16, 32}) {
         for (int r = r_lo; r <= 13; ++r) {
             const int64_t B = (int64_t(1) << r) / A;
-            if (B < 64 || B > 1024 || (kernel != HGM_KERNEL_LDS && B > reg_kernel_max_threads(A))) continue;
+            if (B < 64 || B > 1024 || (kernel != HGM_KERNEL_LDS && B > reg_kernel_max_threads(A, amp_bytes(dtype)))) continue;
             // plan-compiled candidates: runs of 128 and 256 bytes per wave segment
             for (int c = jit ? c_run : 2; c <= (jit ? c_run + 1 : 4); ++c) {
                 hgm_tuning t;
