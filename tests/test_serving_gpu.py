@@ -130,6 +130,47 @@ def test_precompile_while_requests_are_in_flight():
         assert f >= 1 - 1e-6, (j.kind, j.qubits, f)
 
 
+def _fp64_reference(job):
+    """The job's final state through the plain FP64 drop-in path (hgm_execute)."""
+    import ctypes as C
+    psi = np.zeros(1 << job.qubits, np.complex128)
+    psi[job.basis] = 1
+    tun = S.sequential_tuning(job.qubits)
+    st = N.Stats()
+    N.check(N.lib.hgm_execute(psi.ctypes.data, job.qubits, N.HGM_C128, S._op_ptr(job.ops), len(job.ops),
+                              C.byref(tun), 1, C.byref(st)))
+    return psi
+
+
+@pytest.mark.parametrize("compiled", [False, True])
+def test_large_tiles_at_21_to_24_qubits(compiled):
+    """FP32 states of 21-24 qubits default to 2^13 and 2^14 tiles (512 and 1024
+    threads): interpreted and compiled, Grover, random layers and the adder
+    match the FP64 drop-in path."""
+    rng = np.random.default_rng(44)
+    jobs = [S.build_job(0, 0.0, "grover", 21, rng, grover_max_iter=3),
+            S.build_job(1, 0.0, "random", 22, rng),
+            S.build_job(2, 0.0, "grover", 23, rng, grover_max_iter=3),
+            S.build_job(3, 0.0, "adder", 24, rng),
+            S.build_job(4, 0.0, "random", 24, rng)]
+    with S.Server(np.complex64, max_active=8, hbm_budget_gib=4) as srv:
+        assert srv.tuning(22)["gpu_region_bits"] == 13 and srv.tuning(24)["gpu_region_bits"] == 14
+        assert srv.tuning(24)["block_size"] == 1024 and srv.tuning(26)["gpu_region_bits"] == 12
+        if compiled:
+            for n in {j.qubits for j in jobs}:
+                for ops, iters in S.mix_shapes(n):
+                    srv.precompile(n, ops, iters)
+        states, res = run_collect(srv, jobs, np.complex64, together=True)
+    for j in jobs:
+        assert res[j.index]["status"] == 0
+        ref = _fp64_reference(j)
+        f = S.fidelity(ref, states[j.index])
+        say("large tile", compiled, j.kind, j.qubits, f)
+        assert f >= 1 - 1e-6, (j.kind, j.qubits, f)
+        if j.kind == "adder":
+            assert res[j.index]["argmax"] == j.expect
+
+
 def test_served_fp64_matches_oracle():
     jobs = small_mix(10, seed=8, qmin=10, qmax=15)
     with S.Server(np.complex128, max_active=16, hbm_budget_gib=1) as srv:
