@@ -317,8 +317,9 @@ struct Oracle {
     }
 
     // PCpuExecution1 (dgm_par_exec.cpp:82-132). Like the reference it does
-    // not loop over "iterations" (the t_PAR_CPU/t_GPU divergence the survey
-    // calls H4); threads claim the next region id from a shared counter.
+    // not loop over "iterations" (t_PAR_CPU applies the PT list once while
+    // t_GPU replicates it); threads claim the next region id from a shared
+    // counter.
     bool run_tparcpu(int threads, int coalesced, int region_bits) {
         std::vector<std::array<C, 4>> mats(gates.size());
         for (size_t i = 0; i < gates.size(); ++i)
