@@ -284,7 +284,13 @@ int make_shape(int qubits, int dtype, const hgm_tuning &t, TileShape *out, std::
         return HGM_E_TUNING;
     }
     if (t.kernel < HGM_KERNEL_AUTO || t.kernel > HGM_KERNEL_JIT) { *err = "unknown kernel family"; return HGM_E_TUNING; }
-    if ((size_t(1) << r) * amp_bytes(dtype) > kMaxLdsBytes - kLdsReserve) {
+    // the plan-compiled kernels move a tile above 32 KiB through LDS in halves (real parts, then imaginary
+    // parts), with one padding slot per 16 and per 256 entries: FP32 tiles of 2^15 fit
+    const size_t tile_bytes = (size_t(1) << r) * amp_bytes(dtype);
+    const size_t lds_need = t.kernel == HGM_KERNEL_JIT && tile_bytes > 32 * 1024
+                                ? (tile_bytes + (tile_bytes >> 4) + (tile_bytes >> 8)) / 2
+                                : tile_bytes;
+    if (lds_need > kMaxLdsBytes - kLdsReserve) {
         *err = "Erro de tuning: a tile of 2^" + std::to_string(r) + " amplitudes exceeds the 160 KiB LDS of a gfx950 workgroup";
         return HGM_E_TUNING;
     }
@@ -642,7 +648,7 @@ void rotate_rows(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_o
 }
 
 int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, const hgm_tuning &t,
-               Plan *out, std::string *err) {
+               Plan *out, std::string *err, bool program_only) {
     auto t0 = std::chrono::steady_clock::now();
     Plan plan;
     plan.qubits = qubits;
@@ -861,11 +867,17 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
     }
     const int A = 2 * plan.shape.R;
     const bool reg_ok = A <= kMaxRegAmps && plan.shape.B <= reg_kernel_max_threads(A, amp_bytes(dtype));
-    if ((t.kernel == HGM_KERNEL_REG || t.kernel == HGM_KERNEL_JIT) && !reg_ok) {
+    const int max_B = t.kernel == HGM_KERNEL_JIT ? compiled_kernel_max_threads(A, amp_bytes(dtype))
+                                                 : reg_kernel_max_threads(A, amp_bytes(dtype));
+    if ((t.kernel == HGM_KERNEL_REG || t.kernel == HGM_KERNEL_JIT) && (A > kMaxRegAmps || plan.shape.B > max_B)) {
         *err = "Erro de tuning: the register kernel holds at most " + std::to_string(kMaxRegAmps) +
-               " amplitudes per thread and at most " + std::to_string(reg_kernel_max_threads(A, amp_bytes(dtype))) +
-               " threads for 2*repeat_count = " + std::to_string(A);
+               " amplitudes per thread and at most " + std::to_string(max_B) + " threads for 2*repeat_count = " +
+               std::to_string(A);
         return HGM_E_TUNING;
+    }
+    if (program_only && t.kernel != HGM_KERNEL_JIT) {
+        *err = "a program-only plan is for plan-compiled tile shapes (HGM_KERNEL_JIT)";
+        return HGM_E_ARG;
     }
     if (t.kernel == HGM_KERNEL_REG || t.kernel == HGM_KERNEL_JIT || (t.kernel == HGM_KERNEL_AUTO && reg_ok)) {
         plan.kernel = t.kernel == HGM_KERNEL_JIT ? HGM_KERNEL_JIT : HGM_KERNEL_REG;
@@ -878,9 +890,13 @@ int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iter
                        "the 2^32 - 1 of one dispatch; raise repeat_count";
                 return HGM_E_TUNING;
             }
-            plan.jit = std::make_shared<JitCode>();
-            jit_generate(plan, plan.jit.get());
-            if (int rc = jit_compile(plan.jit.get(), err)) return rc;
+            if (program_only) {
+                plan.kernel = HGM_KERNEL_REG;
+            } else {
+                plan.jit = std::make_shared<JitCode>();
+                jit_generate(plan, plan.jit.get());
+                if (int rc = jit_compile(plan.jit.get(), err)) return rc;
+            }
         }
     }
     plan.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -60,6 +60,14 @@ constexpr int reg_kernel_max_threads(int A, size_t amp_bytes) {
     return amp_bytes <= 8 ? (A >= 32 ? 512 : 1024) : (A >= 32 ? 256 : (A >= 16 ? 512 : 1024));
 }
 
+// Workgroup size limit of the plan-compiled pass kernels (and the serving
+// engine's compiled kernels): they keep no interpreter state, so FP32 tiles of
+// 32 amplitudes per thread fit 1024 threads (Grover passes on 2^15-amplitude
+// tiles take 95-128 VGPRs; denser passes may spill). FP64 as the interpreter.
+constexpr int compiled_kernel_max_threads(int A, size_t amp_bytes) {
+    return amp_bytes <= 8 ? 1024 : reg_kernel_max_threads(A, amp_bytes);
+}
+
 struct TileShape {
     int r;      // tile bits
     int B;      // threads per tile
@@ -208,8 +216,11 @@ void fuse_gates(int qubits, const hgm_op *ops, int64_t n_ops, std::vector<hgm_op
 // Builds the pass list for ops[0..n_ops) repeated `iterations` times
 // (after fuse_gates, then pivot_rows or fold_phases, when tuning.planner has
 // HGM_PLAN_FUSE, HGM_PLAN_PIVOT or HGM_PLAN_FOLD_PHASES; fusing first expands
-// the iterations).
+// the iterations). With program_only, a plan-compiled (HGM_KERNEL_JIT)
+// tuning gets its tile shape checked and its register program built, but no
+// kernels generated or compiled: the plan (kernel HGM_KERNEL_REG) is for
+// kernels compiled from its passes' structure elsewhere (the serving engine).
 int build_plan(int qubits, int dtype, const hgm_op *ops, int64_t n_ops, int iterations, const hgm_tuning &t,
-               Plan *out, std::string *err);
+               Plan *out, std::string *err, bool program_only = false);
 
 }  // namespace hgm

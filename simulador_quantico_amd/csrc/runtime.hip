@@ -1091,8 +1091,8 @@ int hgm_plan_jit_source(const hgm_plan *p, char *out, size_t cap, size_t *len) {
 
 int hgm_plan_serving_source(const hgm_plan *p, int64_t pass, char *out, size_t cap, size_t *len) {
     if (!p || !len || pass < 0 || pass >= (int64_t)p->plan.passes.size()) { t_err = "bad arguments"; return HGM_E_ARG; }
-    if (p->plan.kernel != HGM_KERNEL_REG || p->plan.passes[pass].op_count == 0) {
-        t_err = "serving kernels are generated for the non-empty passes of register-kernel plans";
+    if ((p->plan.kernel != HGM_KERNEL_REG && p->plan.kernel != HGM_KERNEL_JIT) || p->plan.passes[pass].op_count == 0) {
+        t_err = "serving kernels are generated for the non-empty passes of register-program plans";
         return HGM_E_ARG;
     }
     size_t lds = 0;

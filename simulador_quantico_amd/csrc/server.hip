@@ -223,6 +223,30 @@ hgm_tuning serving_default_tuning(int n, int dtype) {
     return t;
 }
 
+// FP32 states of 25-26 qubits can also run on tiles of 2^15 amplitudes (1024
+// threads x 32 amplitudes, a 140 KiB LDS remap tile in halves), which only the
+// compiled pass kernels hold. Their n - 4 non-coalesced bits then fit two
+// windows of 11 bits, so Grover takes 2 passes per iteration instead of 4; but
+// one such workgroup fills a CU, so a pass streams slower (Grover at 26
+// qubits: 0.33 against 0.19 ms per pass on 2^12 tiles, kLargeTilePassCost).
+// A request takes this shape only when every pass of its plan has a compiled
+// kernel and the plan has fewer than 1/kLargeTilePassCost of the passes.
+constexpr double kLargeTilePassCost = 1.7;
+
+bool serving_large_tile(int n, int dtype, hgm_tuning *t) {
+    const int c = 4, r = 15;
+    if (dtype != HGM_C64 || n - c <= 2 * (14 - c) || n - c > 2 * (r - c)) return false;
+    hgm_default_tuning(t);
+    t->ops_block = 1 << 30;
+    t->kernel = HGM_KERNEL_JIT;   // shape rules of the compiled kernels; planned program-only
+    t->planner = kCompiledPlanner;
+    t->block_size = 1024;
+    t->repeat_count = 16;
+    t->gpu_region_bits = r;
+    t->gpu_coalesced_bits = c;
+    return true;
+}
+
 // The autotuner's timing circuit for `n` qubits: the serving mix's expected
 // work at that qubit count, one request of each kind (the mix draws the kinds
 // uniformly): a Hadamard layer, Grover with 32 iterations (the mean of 1..64),
@@ -267,6 +291,7 @@ struct hgm_server {
     Lane lanes[kLanes];
     size_t slot_bytes = 0, tasks_off = 0, results_off = 0;
     std::vector<hgm_tuning> tune;             // per qubit count
+    std::vector<hgm_tuning> large;            // per qubit count: the 2^15-tile alternative (block_size 0: none)
 
     std::mutex mu;                            // submissions, results, plan queue
     std::condition_variable cv_work, cv_done, cv_plan;
@@ -319,10 +344,40 @@ void fail(hgm_server *s, Req *r, int status, const std::string &msg) {
     finish(s, r);
 }
 
+// Replans a request on the 2^15-tile alternative when every pass of that plan
+// has a compiled kernel and the plan saves enough passes (kLargeTilePassCost);
+// otherwise leaves the request as it is.
+void try_large_tile(hgm_server *s, Req *r, const hgm_tuning &lt) {
+    {
+        std::lock_guard<std::mutex> g(s->kmu);
+        if (s->kernel_of.empty()) return;
+    }
+    Plan plan;
+    std::string err;
+    if (build_plan(r->qubits, s->cfg.dtype, r->ops.data(), (int64_t)r->ops.size(), r->iterations, lt, &plan, &err,
+                   true) != HGM_OK)
+        return;
+    std::vector<int> passes, kernel;
+    for (size_t i = 0; i < plan.passes.size(); ++i)
+        if (plan.passes[i].op_count > 0) passes.push_back((int)i);
+    if (passes.empty() || (double)passes.size() * kLargeTilePassCost >= (double)r->passes.size()) return;
+    {
+        std::lock_guard<std::mutex> g(s->kmu);
+        for (int i : passes) {
+            auto it = s->kernel_of.find(serving_pass_key(plan, plan.passes[i]));
+            if (it == s->kernel_of.end()) return;
+            kernel.push_back(it->second);
+        }
+    }
+    r->plan = std::move(plan);
+    r->passes = std::move(passes);
+    r->kernel = std::move(kernel);
+}
+
 void planner_loop(hgm_server *s) {
     for (;;) {
         Req *r;
-        hgm_tuning t;
+        hgm_tuning t, lt;
         {
             std::unique_lock<std::mutex> g(s->mu);
             s->cv_plan.wait(g, [&] { return s->stop || !s->plan_q.empty(); });
@@ -330,6 +385,7 @@ void planner_loop(hgm_server *s) {
             r = s->plan_q.front();
             s->plan_q.pop_front();
             t = s->tune[r->qubits];
+            lt = s->large[r->qubits];
         }
         std::string err;
         int rc = build_plan(r->qubits, s->cfg.dtype, r->ops.data(), (int64_t)r->ops.size(), r->iterations, t,
@@ -356,6 +412,7 @@ void planner_loop(hgm_server *s) {
                     if (it != s->kernel_of.end()) r->kernel[i] = it->second;
                 }
         }
+        if (rc == HGM_OK && lt.block_size > 0) try_large_tile(s, r, lt);
         if (rc == HGM_OK) {
             const Plan &p = r->plan;
             r->n_tiles = int64_t(1) << (r->qubits - p.shape.r);
@@ -665,7 +722,11 @@ int hgm_server_create(const hgm_server_config *cfg, hgm_server **out) {
     hipGetDevice(&s->device);
     s->t0 = std::chrono::steady_clock::now();
     s->tune.resize(kMaxQubits + 1);
-    for (int n = 1; n <= kMaxQubits; ++n) s->tune[n] = serving_default_tuning(n, cfg->dtype);
+    s->large.resize(kMaxQubits + 1);
+    for (int n = 1; n <= kMaxQubits; ++n) {
+        s->tune[n] = serving_default_tuning(n, cfg->dtype);
+        if (!serving_large_tile(n, cfg->dtype, &s->large[n])) s->large[n].block_size = 0;
+    }
     if (cfg->autotune) {
         for (int n = std::max(cfg->min_qubits, 14); n <= cfg->max_qubits; ++n) {
             std::vector<hgm_op> ops;
@@ -785,6 +846,7 @@ int hgm_server_set_tuning(hgm_server *s, int qubits, const hgm_tuning *t) {
     if (shape.B > reg_kernel_max_threads(A, amp_bytes(s->cfg.dtype))) { set_error("block_size above the register kernel's limit for this repeat_count"); return HGM_E_TUNING; }
     std::lock_guard<std::mutex> g(s->mu);   // planner threads copy their entry under this lock
     s->tune[qubits] = *t;
+    s->large[qubits].block_size = 0;        // the caller's shape replaces the 2^15-tile alternative
     return HGM_OK;
 }
 
@@ -794,19 +856,32 @@ int hgm_server_precompile(hgm_server *s, int qubits, const hgm_op *ops, int64_t 
         set_error("bad argument");
         return HGM_E_ARG;
     }
-    hgm_tuning t;
+    hgm_tuning t, lt;
     {
         std::lock_guard<std::mutex> g(s->mu);
         t = s->tune[qubits];
+        lt = s->large[qubits];
         if (qubits <= t.gpu_region_bits) return HGM_OK;   // one tile: the interpreter runs the whole circuit
         t.planner = kCompiledPlanner;
         s->tune[qubits] = t;   // requests planned from now on find the kernels
     }
-    Plan plan;
+    Plan plan, large;
     std::string err;
     int rc = build_plan(qubits, s->cfg.dtype, ops, n_ops, iterations, t, &plan, &err);
     if (rc) { set_error(err); return rc; }
     if (plan.kernel != HGM_KERNEL_REG) { set_error("serving needs the register kernel for this tile shape"); return HGM_E_TUNING; }
+    auto busy = [](const Plan &p) {
+        size_t k = 0;
+        for (const Pass &ps : p.passes) k += ps.op_count > 0;
+        return k;
+    };
+    // the 2^15-tile plan's kernels too, where a request of this shape would take that plan
+    bool with_large = false;
+    if (lt.block_size > 0) {
+        rc = build_plan(qubits, s->cfg.dtype, ops, n_ops, iterations, lt, &large, &err, true);
+        if (rc) { set_error(err); return rc; }
+        with_large = (double)busy(large) * kLargeTilePassCost < (double)busy(plan);
+    }
     struct Job {
         std::string key;
         JitCode code;
@@ -818,15 +893,18 @@ int hgm_server_precompile(hgm_server *s, int qubits, const hgm_op *ops, int64_t 
     {
         std::lock_guard<std::mutex> g(s->kmu);
         std::unordered_set<std::string> seen;
-        for (const Pass &ps : plan.passes) {
-            if (!ps.op_count) continue;
-            std::string key = serving_pass_key(plan, ps);
-            if (s->kernel_of.count(key) || !seen.insert(key).second) continue;
-            Job j;
-            j.key = std::move(key);
-            j.code.source = serving_kernel_source(plan, ps, "hgm_serve", &j.lds);
-            j.code.names = {"hgm_serve"};
-            jobs.push_back(std::move(j));
+        for (const Plan *p : {&plan, &large}) {
+            if (p == &large && !with_large) continue;
+            for (const Pass &ps : p->passes) {
+                if (!ps.op_count) continue;
+                std::string key = serving_pass_key(*p, ps);
+                if (s->kernel_of.count(key) || !seen.insert(key).second) continue;
+                Job j;
+                j.key = std::move(key);
+                j.code.source = serving_kernel_source(*p, ps, "hgm_serve", &j.lds);
+                j.code.names = {"hgm_serve"};
+                jobs.push_back(std::move(j));
+            }
         }
     }
     // hipRTC compiles independent programs concurrently

@@ -197,6 +197,21 @@ def test_tuning_contract():
     assert tuning_rc(3) == 0                                            # small states run as one tile
 
 
+def test_compiled_kernel_tile_limits():
+    """The compiled kernels move tiles above 32 KiB through LDS in halves with
+    1/16 + 1/256 padding, so FP32 tiles of 2^15 fit 160 KiB for them and not
+    for the interpreter."""
+    def rc(dtype, kernel, r, B, R):
+        t = N.default_tuning(block_size=B, repeat_count=R, gpu_region_bits=r, gpu_coalesced_bits=4, kernel=kernel)
+        return N.lib.hgm_validate_tuning(26, dtype, C.byref(t))
+    assert rc(N.HGM_C64, N.HGM_KERNEL_JIT, 15, 1024, 16) == 0
+    assert rc(N.HGM_C64, N.HGM_KERNEL_REG, 15, 1024, 16) == 2
+    assert b"160 KiB" in N.lib.hgm_last_error()
+    assert rc(N.HGM_C64, N.HGM_KERNEL_JIT, 16, 1024, 32) == 2         # 2^16 FP32: 279 KiB per half
+    assert rc(N.HGM_C128, N.HGM_KERNEL_JIT, 15, 1024, 16) == 2        # 2^15 FP64: 279 KiB per half
+    assert rc(N.HGM_C128, N.HGM_KERNEL_JIT, 14, 512, 16) == 0         # 2^14 FP64: 139 KiB per half
+
+
 def test_cpu_exec_types_are_rejected():
     d = hq.DGM()
     d.qubits = 3

@@ -171,6 +171,46 @@ def test_large_tiles_at_21_to_24_qubits(compiled):
             assert res[j.index]["argmax"] == j.expect
 
 
+def test_2pow15_tiles_at_25_and_26_qubits():
+    """With compiled kernels, FP32 requests of 25-26 qubits whose plan on 2^15
+    tiles (1024 threads x 32 amplitudes) needs under 1/1.7 of the passes take
+    that plan: Grover with 1-3 iterations (4-8 passes instead of 7-15) and the
+    adder (1 instead of 3); the Hadamard layer and random layers keep 2^12
+    tiles. Results match the FP64 drop-in path, alone and in the mix."""
+    rng = np.random.default_rng(45)
+    jobs = [S.build_job(0, 0.0, "grover", 26, rng, grover_max_iter=3),
+            S.build_job(1, 0.0, "grover", 25, rng, grover_max_iter=3),
+            S.build_job(2, 0.0, "adder", 26, rng),
+            S.build_job(3, 0.0, "random", 26, rng),
+            S.build_job(4, 0.0, "hadamard", 25, rng)]
+    with S.Server(np.complex64, max_active=8, hbm_budget_gib=6) as srv:
+        assert srv.tuning(26)["gpu_region_bits"] == 12
+        interp, ri = run_collect(srv, jobs[:1], np.complex64, together=True)   # no compiled kernels yet
+        for n in (25, 26):
+            for ops, iters in S.mix_shapes(n):
+                srv.precompile(n, ops, iters)
+        alone, ra = run_collect(srv, jobs, np.complex64, together=False)
+        mixed, rm = run_collect(srv, jobs, np.complex64, together=True)
+    say("2^15 tiles passes", [(j.kind, j.qubits, ra[j.index]["passes"]) for j in jobs], "interpreted", ri[0]["passes"])
+    for j in jobs[:2]:   # Grover with k iterations: 3 + 4k passes on 2^12 tiles, 2 + 2k on 2^15 tiles
+        assert ra[j.index]["passes"] == 2 + 2 * j.params["iterations"], j.params
+    assert ri[0]["passes"] == 3 + 4 * jobs[0].params["iterations"]
+    assert ra[2]["passes"] == 1 and ra[4]["passes"] == 3
+    for j in jobs:
+        assert ra[j.index]["status"] == 0 and rm[j.index]["status"] == 0
+        assert rm[j.index]["passes"] == ra[j.index]["passes"]
+        assert np.abs(alone[j.index] - mixed[j.index]).max() <= 1e-7, (j.kind, j.qubits)
+        ref = _fp64_reference(j)
+        f = S.fidelity(ref, alone[j.index])
+        say("2^15 tiles", j.kind, j.qubits, f)
+        assert f >= 1 - 1e-6, (j.kind, j.qubits, f)
+        if j.kind == "adder":
+            assert ra[j.index]["argmax"] == j.expect
+        if j.kind == "grover":
+            assert ra[j.index]["argmax"] == j.params["marked"]
+    assert S.fidelity(_fp64_reference(jobs[0]), interp[0]) >= 1 - 1e-6
+
+
 def test_served_fp64_matches_oracle():
     jobs = small_mix(10, seed=8, qmin=10, qmax=15)
     with S.Server(np.complex128, max_active=16, hbm_budget_gib=1) as srv:
