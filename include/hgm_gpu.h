@@ -266,10 +266,11 @@ int hgm_plan_maps(const hgm_plan *p, void *maps, int64_t *n_maps);
  * may be NULL to query the length. */
 int hgm_plan_jit_source(const hgm_plan *p, char *out, size_t cap, size_t *len);
 /* Source of the serving engine's compiled kernel for pass `pass` of an
- * HGM_KERNEL_REG plan (see hgm_server_precompile): a function of the pass's
- * structure only, so plans of circuits that differ in matrix entries or
- * control values have equal sources. out may be NULL to query the length.
- * No counterpart in the reference (an inspection aid of this library). */
+ * HGM_KERNEL_REG or HGM_KERNEL_JIT plan (see hgm_server_precompile): a
+ * function of the pass's structure only, so plans of circuits that differ in
+ * matrix entries or control values have equal sources. out may be NULL to
+ * query the length. No counterpart in the reference (an inspection aid of
+ * this library). */
 int hgm_plan_serving_source(const hgm_plan *p, int64_t pass, char *out, size_t cap, size_t *len);
 /* Replay the plan's passes from a hipGraph (HGM_KERNEL_JIT plans only): the
  * first hgm_state_run_plan on a state captures its launches, later runs on
@@ -391,8 +392,13 @@ double hgm_server_clock_ms(const hgm_server *s);
  * the same shape with other matrices or control values (another seed, another
  * marked value) finds its passes' kernels; its passes run them, and passes of
  * other structures run the batched interpreter. States of one tile (whose
- * whole circuit is one pass) always run the interpreter. *n_new: kernels
- * added. No counterpart in the reference (the serving engine is this
+ * whole circuit is one pass) always run the interpreter. FP32 requests of
+ * 25-26 qubits are also planned on tiles of 2^15 amplitudes (1024 threads x
+ * 32, which only compiled kernels hold); a request takes that plan when each
+ * of its passes has a kernel and it needs under 1/1.7 of the default plan's
+ * passes, and precompile compiles those kernels where the example would take
+ * it. hgm_server_set_tuning for a qubit count turns this off for it. *n_new:
+ * kernels added. No counterpart in the reference (the serving engine is this
  * library's own). */
 int hgm_server_precompile(hgm_server *s, int qubits, const hgm_op *ops, int64_t n_ops, int iterations,
                           int *n_new);
