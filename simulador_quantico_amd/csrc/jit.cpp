@@ -310,7 +310,7 @@ std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const st
     o += "        const unsigned long long tid = blockIdx.x;\n";
     o += "        const unsigned long long base = " + base_expr(nonlocal) + ";\n";
     for (int k = 0; k < A; ++k)
-        o += "        " + g.V + " " + reg(k) + " = s[(base | " + hex64(pm.lreg[k]) + ") + lt];\n";
+        o += "        " + g.V + " " + reg(k) + " = hgm_ld(&s[(base | " + hex64(pm.lreg[k]) + ") + lt]);\n";
     int cur = m_first;
     std::vector<RegOp> ops(plan.rops.begin() + ps.rop_begin, plan.rops.begin() + ps.rop_begin + ps.rop_count);
     schedule_waves(ops);
@@ -403,7 +403,7 @@ std::string kernel_body(const Gen &g, const Plan &plan, const Pass &ps, const st
     flush_factors();
     flush_scales();
     for (int k = 0; k < A; ++k)
-        o += "        s[(base | " + hex64(pm.sreg[k]) + ") + st] = " + reg(k) + ";\n";
+        o += "        hgm_st(&s[(base | " + hex64(pm.sreg[k]) + ") + st], " + reg(k) + ");\n";
     o += "    }\n}\n";
     return o;
 }
@@ -460,9 +460,19 @@ void jit_generate(const Plan &plan, JitCode *out) {
     const size_t entries = (size_t)padded((uint32_t)last) + 1;
     g.split = entries * amp_bytes(plan.dtype) > 32 * 1024;
     out->lds_bytes = entries * amp_bytes(plan.dtype) / (g.split ? 2 : 1);
+    // A pass reads and writes every amplitude once and nothing re-reads it
+    // before the next pass: loads and stores carry the nontemporal (streaming)
+    // hint (random30 FP64 99.4-99.9 -> 96.7-97.1 ms per step, random32 FP32
+    // 221 -> 215 ms, on one box).
     out->source = "// generated pass kernels\n"
                   "struct __attribute__((aligned(16))) hgm_d2 { double x, y; };\n"
-                  "struct __attribute__((aligned(8))) hgm_f2 { float x, y; };\n";
+                  "struct __attribute__((aligned(8))) hgm_f2 { float x, y; };\n"
+                  "typedef double hgm_v2d __attribute__((ext_vector_type(2)));\n"
+                  "typedef float hgm_v2f __attribute__((ext_vector_type(2)));\n"
+                  "__device__ inline hgm_d2 hgm_ld(const hgm_d2 *p) { hgm_v2d v = __builtin_nontemporal_load((const hgm_v2d *)p); return {v.x, v.y}; }\n"
+                  "__device__ inline hgm_f2 hgm_ld(const hgm_f2 *p) { hgm_v2f v = __builtin_nontemporal_load((const hgm_v2f *)p); return {v.x, v.y}; }\n"
+                  "__device__ inline void hgm_st(hgm_d2 *p, hgm_d2 a) { hgm_v2d v = {a.x, a.y}; __builtin_nontemporal_store(v, (hgm_v2d *)p); }\n"
+                  "__device__ inline void hgm_st(hgm_f2 *p, hgm_f2 a) { hgm_v2f v = {a.x, a.y}; __builtin_nontemporal_store(v, (hgm_v2f *)p); }\n";
     out->pass_kernel.clear();
     out->names.clear();
     std::map<std::string, int> seen;                 // body (with a placeholder name) -> kernel index
