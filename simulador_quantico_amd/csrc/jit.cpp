@@ -109,6 +109,27 @@ uint32_t unswizzle(uint32_t x) { return x ^ ((x >> 4) & 15u) ^ ((x >> 8) & 15u) 
 // per-thread part plus a per-register constant (an immediate offset).
 uint32_t padded(uint32_t x) { return x + (x >> 4) + (x >> 8); }
 
+// Amplitude types of the generated kernels and their state accessors. A pass
+// of the plan-compiled kernels reads and writes every amplitude once and
+// nothing re-reads it before the next pass: their loads and stores carry the
+// nontemporal (streaming) hint (random30 FP64 99.4-99.9 -> 96.7-97.1 ms per
+// step, random32 FP32 221 -> 215 ms, on one box). The serving kernels do not:
+// many of the mix's states fit the last-level cache between steps, and the
+// hint cost 0.5% of the mix's circuits/s (1086-1089 against 1093).
+const char *const kAmpTypes =
+    "struct __attribute__((aligned(16))) hgm_d2 { double x, y; };\n"
+    "struct __attribute__((aligned(8))) hgm_f2 { float x, y; };\n"
+    "typedef double hgm_v2d __attribute__((ext_vector_type(2)));\n"
+    "typedef float hgm_v2f __attribute__((ext_vector_type(2)));\n"
+    "__device__ inline hgm_d2 hgm_ld(const hgm_d2 *p) {\n"
+    "    hgm_v2d v = __builtin_nontemporal_load((const hgm_v2d *)p); return {v.x, v.y}; }\n"
+    "__device__ inline hgm_f2 hgm_ld(const hgm_f2 *p) {\n"
+    "    hgm_v2f v = __builtin_nontemporal_load((const hgm_v2f *)p); return {v.x, v.y}; }\n"
+    "__device__ inline void hgm_st(hgm_d2 *p, hgm_d2 a) {\n"
+    "    hgm_v2d v = {a.x, a.y}; __builtin_nontemporal_store(v, (hgm_v2d *)p); }\n"
+    "__device__ inline void hgm_st(hgm_f2 *p, hgm_f2 a) {\n"
+    "    hgm_v2f v = {a.x, a.y}; __builtin_nontemporal_store(v, (hgm_v2f *)p); }\n";
+
 // base = deposit of the tile id into the non-local state bits, by runs
 std::string base_expr(uint64_t nonlocal) {
     std::string s;
@@ -460,19 +481,7 @@ void jit_generate(const Plan &plan, JitCode *out) {
     const size_t entries = (size_t)padded((uint32_t)last) + 1;
     g.split = entries * amp_bytes(plan.dtype) > 32 * 1024;
     out->lds_bytes = entries * amp_bytes(plan.dtype) / (g.split ? 2 : 1);
-    // A pass reads and writes every amplitude once and nothing re-reads it
-    // before the next pass: loads and stores carry the nontemporal (streaming)
-    // hint (random30 FP64 99.4-99.9 -> 96.7-97.1 ms per step, random32 FP32
-    // 221 -> 215 ms, on one box).
-    out->source = "// generated pass kernels\n"
-                  "struct __attribute__((aligned(16))) hgm_d2 { double x, y; };\n"
-                  "struct __attribute__((aligned(8))) hgm_f2 { float x, y; };\n"
-                  "typedef double hgm_v2d __attribute__((ext_vector_type(2)));\n"
-                  "typedef float hgm_v2f __attribute__((ext_vector_type(2)));\n"
-                  "__device__ inline hgm_d2 hgm_ld(const hgm_d2 *p) { hgm_v2d v = __builtin_nontemporal_load((const hgm_v2d *)p); return {v.x, v.y}; }\n"
-                  "__device__ inline hgm_f2 hgm_ld(const hgm_f2 *p) { hgm_v2f v = __builtin_nontemporal_load((const hgm_v2f *)p); return {v.x, v.y}; }\n"
-                  "__device__ inline void hgm_st(hgm_d2 *p, hgm_d2 a) { hgm_v2d v = {a.x, a.y}; __builtin_nontemporal_store(v, (hgm_v2d *)p); }\n"
-                  "__device__ inline void hgm_st(hgm_f2 *p, hgm_f2 a) { hgm_v2f v = {a.x, a.y}; __builtin_nontemporal_store(v, (hgm_v2f *)p); }\n";
+    out->source = std::string("// generated pass kernels\n") + kAmpTypes;
     out->pass_kernel.clear();
     out->names.clear();
     std::map<std::string, int> seen;                 // body (with a placeholder name) -> kernel index
@@ -657,9 +666,7 @@ std::string serving_kernel_source(const Plan &plan, const Pass &ps, const std::s
         st[j] = pm.sthr[j / 5][1 << (j % 5)];
     }
     std::string o;
-    o += "// generated serving pass kernel\n"
-         "struct __attribute__((aligned(16))) hgm_d2 { double x, y; };\n"
-         "struct __attribute__((aligned(8))) hgm_f2 { float x, y; };\n"
+    o += std::string("// generated serving pass kernel\n") + kAmpTypes +
          "struct hgm_item { unsigned long long state_off, ops_off, maps_off, pm_off, nonlocal_mask;\n"
          "                  long long tile_begin, partials_off; int n_ops, pad; };\n"
          "struct hgm_rop { int kind, rbit; unsigned rmask, rval, tmask, tval; unsigned long long gmask, gval;\n"
