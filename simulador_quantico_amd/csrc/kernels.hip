@@ -18,6 +18,24 @@ template <class T> struct Vec;
 template <> struct Vec<double> { using type = double2; };
 template <> struct Vec<float> { using type = float2; };
 
+// State accesses of a single-circuit pass, which reads and writes every
+// amplitude once per pass: nontemporal (streaming) hint (interpreted random30
+// FP64 609 -> 578 ms per step, Hadamard x3 at 30 qubits 53.6 -> 52.7 ms, on
+// one box). The batched serving kernel keeps plain accesses: many of its
+// states fit the last-level cache between steps.
+template <class T>
+__device__ __forceinline__ typename Vec<T>::type load_stream(const typename Vec<T>::type *p) {
+    typedef T N __attribute__((ext_vector_type(2)));
+    const N v = __builtin_nontemporal_load(reinterpret_cast<const N *>(p));
+    return {v.x, v.y};
+}
+template <class T>
+__device__ __forceinline__ void store_stream(typename Vec<T>::type *p, const typename Vec<T>::type &a) {
+    typedef T N __attribute__((ext_vector_type(2)));
+    const N v = {a.x, a.y};
+    __builtin_nontemporal_store(v, reinterpret_cast<N *>(p));
+}
+
 __device__ __forceinline__ uint64_t pdep64(uint64_t v, uint64_t mask) {
     uint64_t r = 0;
     for (uint64_t bb = 1; mask; bb <<= 1) {
@@ -401,7 +419,7 @@ __global__ void __launch_bounds__(reg_kernel_max_threads(A, sizeof(typename Vec<
         const uint64_t base = pdep64((uint64_t)tid, nonlocal_mask);
         V a[A];
 #pragma unroll
-        for (int k = 0; k < A; ++k) a[k] = state[base | lt | pm->lreg[k]];
+        for (int k = 0; k < A; ++k) a[k] = load_stream<T>(&state[base | lt | pm->lreg[k]]);
         int cur = pm->first_map;
         for (int i = 0; i < n_ops; ++i) {
             const RegOp &o = ops[i];
@@ -415,7 +433,7 @@ __global__ void __launch_bounds__(reg_kernel_max_threads(A, sizeof(typename Vec<
             else reg_gate<T, A, false>(a, o, t);
         }
 #pragma unroll
-        for (int k = 0; k < A; ++k) state[base | st | pm->sreg[k]] = a[k];
+        for (int k = 0; k < A; ++k) store_stream<T>(&state[base | st | pm->sreg[k]], a[k]);
     }
 }
 
