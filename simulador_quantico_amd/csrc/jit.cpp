@@ -114,8 +114,8 @@ uint32_t padded(uint32_t x) { return x + (x >> 4) + (x >> 8); }
 // nothing re-reads it before the next pass: their loads and stores carry the
 // nontemporal (streaming) hint (random30 FP64 99.4-99.9 -> 96.7-97.1 ms per
 // step, random32 FP32 221 -> 215 ms, on one box). The serving kernels do not:
-// many of the mix's states fit the last-level cache between steps, and the
-// hint cost 0.5% of the mix's circuits/s (1086-1089 against 1093).
+// with the hint the mix served 1086-1089 against 1093 circuits/s, and with it
+// on states above 256 MiB only (26 qubits) 1066-1068 against 1088-1090.
 const char *const kAmpTypes =
     "struct __attribute__((aligned(16))) hgm_d2 { double x, y; };\n"
     "struct __attribute__((aligned(8))) hgm_f2 { float x, y; };\n"
