@@ -280,6 +280,24 @@ int hgm_plan_set_graph(hgm_plan *p, int enable);
  * came from the process cache or the plan is not JIT). */
 double hgm_plan_compile_seconds(const hgm_plan *p);
 
+/* A batch of n resident states of any qubit counts, state i advanced by the
+ * HGM_KERNEL_JIT plan plans[i], captured as one hipGraph: the graph forks
+ * into up to 4 branches (states spread over them by their plans' bytes,
+ * largest first), each branch runs its states' passes in order, and the
+ * branches join, so passes of different states can run at once. One
+ * hgm_batch_run is one graph launch for the whole batch. States must be
+ * distinct and match their plan's qubit count and dtype; the plans' compiled
+ * modules (at most 16 distinct) are kept loaded while the batch runs, and the
+ * graph is captured again when one was unloaded in between. The states and
+ * plans must outlive the batch. No counterpart in the reference (it runs one
+ * state per call: src/core/kernel.cu:192-282). */
+typedef struct hgm_batch hgm_batch;
+int hgm_batch_create(hgm_state *const *states, const hgm_plan *const *plans, int n, hgm_batch **out);
+/* Launches the batch's graph and waits for it; stats sums the plans' gates,
+ * passes and bytes, with the graph's device time as kernel_ms. */
+int hgm_batch_run(hgm_batch *b, hgm_stats *stats);
+void hgm_batch_destroy(hgm_batch *b);
+
 /* The gate list after phase folding (HGM_PLAN_FOLD_PHASES), for inspection
  * and checking: out may be NULL to query *n_out (at most 2 * n_ops + qubits). */
 int hgm_fold_phases(int qubits, const hgm_op *ops, int64_t n_ops, hgm_op *out, int64_t cap, int64_t *n_out);

@@ -10,7 +10,7 @@ from .circuits import (AddF, C2AddF, C2AddMod, C2SubF, C2SubMod, CAddF, CMultMod
                        ControlledN, ControledZ, CRMultMod, CSubF, CSwapR, DraperAdder, GroverIteration,
                        GroverIterationRef, GroverIterations, Hadamard, Oracle1, Pauli_X, Pauli_Z, QFT, RandomLayers,
                        RQFT, Shor, SubF, SwapOver, Toffoli, genRot)
-from .dgm import (DENSE, DIAG_PRI, DIAG_SEC, DGM, T_CPU, T_GPU, T_HYBRID, T_PAR_CPU, Plan, State, make_ops,
+from .dgm import (DENSE, DIAG_PRI, DIAG_SEC, DGM, T_CPU, T_GPU, T_HYBRID, T_PAR_CPU, Batch, Plan, State, make_ops,
                   ops_to_arrays, seed_rng)
 
 __all__ = [name for name in dir() if not name.startswith("_")]

@@ -131,6 +131,9 @@ SIGNATURES = [
     ("hgm_plan_jit_source", C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     ("hgm_plan_compile_seconds", C.c_double, [_P]),
     ("hgm_plan_set_graph", C.c_int, [_P, C.c_int]),
+    ("hgm_batch_create", C.c_int, [C.POINTER(_P), C.POINTER(_P), C.c_int, C.POINTER(_P)]),
+    ("hgm_batch_run", C.c_int, [_P, C.POINTER(Stats)]),
+    ("hgm_batch_destroy", None, [_P]),
     ("hgm_validate_tuning", C.c_int, [C.c_int, C.c_int, C.POINTER(Tuning)]),
     ("hgm_fold_phases", C.c_int, [C.c_int, C.POINTER(Op), C.c_int64, C.POINTER(Op), C.c_int64,
                                   C.POINTER(C.c_int64)]),

@@ -54,6 +54,26 @@ struct hgm_plan {
     }
 };
 
+// A batch of states advanced by plan-compiled plans as one captured graph
+// (hgm_batch_create): branch[i] is the fork branch of state i.
+struct hgm_batch {
+    std::vector<hgm_state *> states;
+    std::vector<const hgm_plan *> plans;
+    std::vector<int> branch;
+    int n_branches = 0;
+    hipStream_t streams[4] = {};
+    hipEvent_t fork = nullptr, join[4] = {};
+    std::vector<uint64_t> gens;   // load generation of each plan's module when the graph was captured
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void drop_graph() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
 namespace hgm {
 
 namespace {
@@ -243,8 +263,9 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 // One pass of a plan-compiled plan: one tile per workgroup, LDS slots padded
-// by 1/16 + 1/256. Assumes the lock.
-int launch_jit_pass_locked(hgm_state *s, const Plan &p, JitModule *jm, size_t pi) {
+// by 1/16 + 1/256. On the context stream unless `stream` is given. Assumes
+// the lock.
+int launch_jit_pass_locked(hgm_state *s, const Plan &p, JitModule *jm, size_t pi, hipStream_t stream = nullptr) {
     Ctx &c = ctx();
     const Pass &ps = p.passes[pi];
     const size_t jit_lds = p.jit->lds_bytes;
@@ -252,7 +273,8 @@ int launch_jit_pass_locked(hgm_state *s, const Plan &p, JitModule *jm, size_t pi
     void *state = s->d;
     void *args[] = {&state};
     HGM_HIP(hipModuleLaunchKernel(jm->fns[p.jit->pass_kernel[pi]], (unsigned)n_tiles, 1, 1, (unsigned)p.shape.B, 1,
-                                  1, ps.phases > 1 ? (unsigned)jit_lds : 0u, c.stream, args, nullptr),
+                                  1, ps.phases > 1 ? (unsigned)jit_lds : 0u, stream ? stream : c.stream, args,
+                                  nullptr),
             "pass launch");
     return HGM_OK;
 }
@@ -423,6 +445,58 @@ int prob_halves_locked(const hgm_state *s, int bit, double *zero, double *one) {
     for (int b = 0; b < blocks; ++b) { z += c.h_partial[2 * b]; o += c.h_partial[2 * b + 1]; }
     *zero = z;
     *one = o;
+    return HGM_OK;
+}
+
+// Loads the module of every plan of the batch and reports whether any load
+// generation differs from the one its graph was captured with. Loading
+// touches each module, so the batch's (at most kMaxJitModules distinct)
+// modules are the most recently used ones and loading one never unloads
+// another of them. Assumes the lock.
+int batch_modules_locked(hgm_batch *b, std::vector<JitModule *> *jms, bool *stale) {
+    jms->assign(b->plans.size(), nullptr);
+    for (size_t i = 0; i < b->plans.size(); ++i)
+        if (int rc = jit_load_locked(*b->plans[i]->plan.jit, &(*jms)[i])) return rc;
+    *stale = !b->exec || b->gens.size() != b->plans.size();
+    for (size_t i = 0; i < b->plans.size() && !*stale; ++i) *stale = b->gens[i] != (*jms)[i]->gen;
+    return HGM_OK;
+}
+
+// Captures the batch: a fork on the context stream, each state's passes in
+// order on its branch stream, a join back. Assumes the lock.
+int capture_batch_locked(hgm_batch *b, const std::vector<JitModule *> &jms) {
+    Ctx &c = ctx();
+    b->drop_graph();
+    HGM_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    int rc = HGM_OK;
+    hipError_t e = hipEventRecord(b->fork, c.stream);
+    for (int k = 0; k < b->n_branches && e == hipSuccess; ++k) e = hipStreamWaitEvent(b->streams[k], b->fork, 0);
+    for (size_t i = 0; i < b->plans.size() && e == hipSuccess && !rc; ++i) {
+        const Plan &p = b->plans[i]->plan;
+        for (size_t pi = 0; pi < p.passes.size() && !rc; ++pi)
+            if (p.passes[pi].op_count)
+                rc = launch_jit_pass_locked(b->states[i], p, jms[i], pi, b->streams[b->branch[i]]);
+    }
+    for (int k = 0; k < b->n_branches && e == hipSuccess && !rc; ++k) {
+        e = hipEventRecord(b->join[k], b->streams[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(c.stream, b->join[k], 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(c.stream, &g);
+    if (rc || e != hipSuccess || ee != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return rc ? rc : hip_fail(e != hipSuccess ? e : ee, "batch graph capture");
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        hipGraphDestroy(g);
+        return hip_fail(ei, "hipGraphInstantiate");
+    }
+    b->graph = g;
+    b->exec = x;
+    b->gens.resize(b->plans.size());
+    for (size_t i = 0; i < b->plans.size(); ++i) b->gens[i] = jms[i]->gen;
     return HGM_OK;
 }
 
@@ -1132,6 +1206,120 @@ int hgm_state_run_plan(hgm_state *s, const hgm_plan *p, hgm_stats *stats) {
     if (!rc) rc = p->graph_on ? run_plan_graph_locked(s, p, stats) : run_plan_locked(s, p->plan, stats, p->plan.plan_ms);
     if (stats) stats->total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+}
+
+int hgm_batch_create(hgm_state *const *states, const hgm_plan *const *plans, int n, hgm_batch **out) {
+    if (!out) { t_err = "null output"; return HGM_E_ARG; }
+    *out = nullptr;
+    if (!states || !plans || n < 1) { t_err = "bad arguments"; return HGM_E_ARG; }
+    std::vector<const void *> seen;
+    std::vector<uint64_t> mods;
+    for (int i = 0; i < n; ++i) {
+        if (!states[i] || !plans[i]) { t_err = "null state or plan"; return HGM_E_ARG; }
+        const Plan &p = plans[i]->plan;
+        if (p.kernel != HGM_KERNEL_JIT || !p.jit) {
+            t_err = "a batch graph runs plan-compiled plans (HGM_KERNEL_JIT)";
+            return HGM_E_UNSUPPORTED;
+        }
+        if (states[i]->qubits != p.qubits || states[i]->dtype != p.dtype) {
+            t_err = "plan was built for another qubit count or dtype";
+            return HGM_E_STATE;
+        }
+        if (std::find(seen.begin(), seen.end(), states[i]->d) != seen.end()) {
+            t_err = "a state appears twice in the batch";
+            return HGM_E_ARG;
+        }
+        seen.push_back(states[i]->d);
+        if (std::find(mods.begin(), mods.end(), p.jit->hash) == mods.end()) mods.push_back(p.jit->hash);
+    }
+    if (mods.size() > kMaxJitModules) {
+        t_err = "a batch graph holds at most " + std::to_string(kMaxJitModules) + " distinct compiled modules";
+        return HGM_E_ARG;
+    }
+    auto b = std::make_unique<hgm_batch>();
+    b->states.assign(states, states + n);
+    b->plans.assign(plans, plans + n);
+    // states by bytes moved, largest first, each to the branch with the fewest so far
+    b->n_branches = std::min(n, 4);
+    std::vector<int> order(n);
+    std::vector<double> bytes(n), load(b->n_branches, 0.0);
+    for (int i = 0; i < n; ++i) {
+        const Plan &p = plans[i]->plan;
+        int64_t busy = 0;
+        for (const Pass &ps : p.passes) busy += ps.op_count > 0;
+        bytes[i] = (double)busy * (double)(uint64_t(1) << p.qubits) * (double)amp_bytes(p.dtype);
+        order[i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return bytes[a] > bytes[c]; });
+    b->branch.assign(n, 0);
+    for (int i : order) {
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        b->branch[i] = k;
+        load[k] += bytes[i];
+    }
+    std::lock_guard<std::mutex> g(ctx().mu);
+    int rc = init_locked(-1);
+    if (rc) return rc;
+    auto release = [&] {
+        b->drop_graph();
+        for (hipStream_t &st : b->streams)
+            if (st) { hipStreamDestroy(st); st = nullptr; }
+        for (hipEvent_t *ev : {&b->fork, &b->join[0], &b->join[1], &b->join[2], &b->join[3]})
+            if (*ev) { hipEventDestroy(*ev); *ev = nullptr; }
+    };
+    hipError_t e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    for (int k = 0; k < b->n_branches && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&b->streams[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "batch streams");
+        release();
+        return rc;
+    }
+    std::vector<JitModule *> jms;
+    bool stale = true;
+    rc = batch_modules_locked(b.get(), &jms, &stale);
+    if (!rc) rc = capture_batch_locked(b.get(), jms);
+    if (rc) {
+        release();
+        return rc;
+    }
+    *out = b.release();
+    return HGM_OK;
+}
+
+int hgm_batch_run(hgm_batch *b, hgm_stats *stats) {
+    if (!b) { t_err = "null batch"; return HGM_E_ARG; }
+    auto t0 = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> g(ctx().mu);
+    int rc = init_locked(-1);
+    if (rc) return rc;
+    Ctx &c = ctx();
+    std::vector<JitModule *> jms;
+    bool stale = false;
+    if ((rc = batch_modules_locked(b, &jms, &stale))) return rc;
+    if (stale && (rc = capture_batch_locked(b, jms))) return rc;
+    HGM_HIP(hipEventRecord(c.e0, c.stream), "hipEventRecord");
+    HGM_HIP(hipGraphLaunch(b->exec, c.stream), "hipGraphLaunch");
+    HGM_HIP(hipEventRecord(c.e1, c.stream), "hipEventRecord");
+    HGM_HIP(hipEventSynchronize(c.e1), "batch execution");
+    const float ms = elapsed(c.e0, c.e1);
+    for (size_t i = 0; i < b->plans.size(); ++i) add_stats(stats, b->plans[i]->plan, i == 0 ? ms : 0.0f, 0.0);
+    if (stats) stats->total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return HGM_OK;
+}
+
+void hgm_batch_destroy(hgm_batch *b) {
+    if (!b) return;
+    std::lock_guard<std::mutex> g(ctx().mu);
+    if (ctx().stream) hipStreamSynchronize(ctx().stream);
+    b->drop_graph();
+    for (hipStream_t st : b->streams)
+        if (st) hipStreamDestroy(st);
+    for (hipEvent_t ev : {b->fork, b->join[0], b->join[1], b->join[2], b->join[3]})
+        if (ev) hipEventDestroy(ev);
+    delete b;
 }
 
 int hgm_plan_set_graph(hgm_plan *p, int enable) {

@@ -407,6 +407,34 @@ class Plan:
         return maps
 
 
+class Batch:
+    """States of any qubit counts, each advanced by its plan-compiled plan,
+    replayed as one captured hipGraph (hgm_batch_*): run() is one graph launch
+    for the whole batch. Keeps references to the states and plans."""
+
+    def __init__(self, states, plans):
+        if len(states) != len(plans):
+            raise ValueError("one plan per state")
+        self._states, self._plans = list(states), list(plans)
+        n = len(self._states)
+        sh = (C.c_void_p * max(1, n))(*[s._h for s in self._states])
+        ph = (C.c_void_p * max(1, n))(*[p._h for p in self._plans])
+        h = C.c_void_p()
+        check(lib.hgm_batch_create(sh, ph, n, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.hgm_batch_destroy(h)
+            self._h = None
+
+    def run(self) -> dict:
+        st = Stats()
+        check(lib.hgm_batch_run(self._h, C.byref(st)))
+        return st.as_dict()
+
+
 # record layouts of csrc/planner.hpp (RegOp, PhaseMap)
 REGOP_DTYPE = np.dtype([("kind", np.int32), ("rbit", np.int32), ("rmask", np.uint32), ("rval", np.uint32),
                         ("tmask", np.uint32), ("tval", np.uint32), ("gmask", np.uint64), ("gval", np.uint64),
