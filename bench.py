@@ -339,6 +339,49 @@ def shor_attempts(number: int, attempts: int, dtype, resident: bool, tuning: dic
             "seconds_first": times[0], "gates_per_attempt": gates}
 
 
+def batch_graph_bench(copies: int = 8, iterations: int = 8, replays: int = 5) -> dict:
+    """Resident FP32 states of 12-19 qubits (`copies` states per qubit count,
+    Grover with `iterations` iterations, plan-compiled on 2^12 tiles), run
+    once as one batch graph (hgm_batch_*) and once plan by plan."""
+    import simulador_quantico_amd as hq
+    from simulador_quantico_amd import _native as N
+    from simulador_quantico_amd.serving import _circuit_ops
+    plans, states = [], []
+    for n in range(12, 20):
+        a = np.concatenate([_circuit_ops([hq.Hadamard(n, 0, n)])] +
+                           [_circuit_ops(hq.GroverIteration(n, ((1 << n) - 1) // 3))] * iterations)
+        ops = (N.Op * len(a)).from_buffer_copy(a.tobytes())
+        t = N.default_tuning(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=4,
+                             kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | N.HGM_PLAN_FUSE, ops_block=1 << 30)
+        p = hq.Plan(n, ops, t, 1, np.complex64)
+        for _ in range(copies):
+            plans.append(p)
+            states.append(hq.State(n, np.complex64))
+    for s in states:
+        s.set_basis(0)
+    for p, s in zip(plans, states):      # warm-up: module loads
+        p.run(s)
+    seq = []
+    for _ in range(replays):
+        t0 = time.perf_counter()
+        for p, s in zip(plans, states):
+            p.run(s)
+        seq.append((time.perf_counter() - t0) * 1e3)
+    b = hq.Batch(states, plans)
+    b.run()
+    wall, dev = [], []
+    for _ in range(replays):
+        t0 = time.perf_counter()
+        st = b.run()
+        wall.append((time.perf_counter() - t0) * 1e3)
+        dev.append(st["kernel_ms"])
+    passes = st["passes"]
+    return {"states": len(states), "qubits": "12-19", "circuit": f"Grover, {iterations} iterations", "dtype": "f32",
+            "passes_per_run": passes, "one_after_another_ms": float(np.median(seq)),
+            "batch_graph_ms": float(np.median(wall)), "batch_graph_device_ms": float(np.median(dev)),
+            "speedup": float(np.median(seq) / np.median(wall))}
+
+
 def serving_bench(requests: int, rate: float, seq_sample: int, seed: int = 2026, autotune: bool = False,
                   fidelity_sample: int = 24, tile=None, planner=None, compiled: bool = True) -> dict:
     """The serving mix on the batched server (FP32 states) against sequential
@@ -487,6 +530,8 @@ def main():
     ap.add_argument("--drop-in", action="store_true",
                     help="run only the workload through the drop-in executor (PCIe-inclusive rate)")
     ap.add_argument("--no-fidelity", action="store_true", help="skip the FP32-vs-FP64 leg of the default run")
+    ap.add_argument("--batch-graph", action="store_true",
+                    help="time 64 resident states of 12-19 qubits as one batch graph against plan by plan")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the checks of the Hadamard, Grover and Draper configurations in the default run")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -541,6 +586,10 @@ def main():
 
     if args.fidelity:
         print(json.dumps({"fidelity_fp32_vs_fp64": fidelity_report()}))
+        return
+
+    if args.batch_graph:
+        print(json.dumps({"batch_graph": batch_graph_bench()}))
         return
 
     if args.serving:

@@ -56,13 +56,14 @@ struct hgm_plan {
 
 // A batch of states advanced by plan-compiled plans as one captured graph
 // (hgm_batch_create): branch[i] is the fork branch of state i.
+constexpr int kBatchBranches = 4;
 struct hgm_batch {
     std::vector<hgm_state *> states;
     std::vector<const hgm_plan *> plans;
     std::vector<int> branch;
     int n_branches = 0;
-    hipStream_t streams[4] = {};
-    hipEvent_t fork = nullptr, join[4] = {};
+    hipStream_t streams[kBatchBranches] = {};
+    hipEvent_t fork = nullptr, join[kBatchBranches] = {};
     std::vector<uint64_t> gens;   // load generation of each plan's module when the graph was captured
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1240,7 +1241,7 @@ int hgm_batch_create(hgm_state *const *states, const hgm_plan *const *plans, int
     b->states.assign(states, states + n);
     b->plans.assign(plans, plans + n);
     // states by bytes moved, largest first, each to the branch with the fewest so far
-    b->n_branches = std::min(n, 4);
+    b->n_branches = std::min(n, kBatchBranches);
     std::vector<int> order(n);
     std::vector<double> bytes(n), load(b->n_branches, 0.0);
     for (int i = 0; i < n; ++i) {
@@ -1264,8 +1265,9 @@ int hgm_batch_create(hgm_state *const *states, const hgm_plan *const *plans, int
         b->drop_graph();
         for (hipStream_t &st : b->streams)
             if (st) { hipStreamDestroy(st); st = nullptr; }
-        for (hipEvent_t *ev : {&b->fork, &b->join[0], &b->join[1], &b->join[2], &b->join[3]})
-            if (*ev) { hipEventDestroy(*ev); *ev = nullptr; }
+        for (hipEvent_t &ev : b->join)
+            if (ev) { hipEventDestroy(ev); ev = nullptr; }
+        if (b->fork) { hipEventDestroy(b->fork); b->fork = nullptr; }
     };
     hipError_t e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
     for (int k = 0; k < b->n_branches && e == hipSuccess; ++k) {
@@ -1317,8 +1319,9 @@ void hgm_batch_destroy(hgm_batch *b) {
     b->drop_graph();
     for (hipStream_t st : b->streams)
         if (st) hipStreamDestroy(st);
-    for (hipEvent_t ev : {b->fork, b->join[0], b->join[1], b->join[2], b->join[3]})
+    for (hipEvent_t ev : b->join)
         if (ev) hipEventDestroy(ev);
+    if (b->fork) hipEventDestroy(b->fork);
     delete b;
 }
 
