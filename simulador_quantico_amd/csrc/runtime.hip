@@ -55,8 +55,10 @@ struct hgm_plan {
 };
 
 // A batch of states advanced by plan-compiled plans as one captured graph
-// (hgm_batch_create): branch[i] is the fork branch of state i.
-constexpr int kBatchBranches = 4;
+// (hgm_batch_create): branch[i] is the fork branch of state i. 64 states of
+// 12-19 qubits (Grover, 1016 passes) replay in 3.78 ms on 4 branches, 3.14 ms
+// on 16 and 2.91 ms on 64, against 7.1 ms plan by plan.
+constexpr int kBatchBranches = 64;
 struct hgm_batch {
     std::vector<hgm_state *> states;
     std::vector<const hgm_plan *> plans;
