@@ -281,15 +281,16 @@ int hgm_plan_set_graph(hgm_plan *p, int enable);
 double hgm_plan_compile_seconds(const hgm_plan *p);
 
 /* A batch of n resident states of any qubit counts, state i advanced by the
- * HGM_KERNEL_JIT plan plans[i], captured as one hipGraph: the graph forks
+ * plan plans[i] (any kernel family: the tables of register and LDS plans are
+ * copied to the device once, here), captured as one hipGraph: the graph forks
  * into up to 64 branches (one per state; beyond 64 states, spread over them
  * by their plans' bytes, largest first), each branch runs its states' passes
  * in order, and the branches join, so passes of different states can run at
  * once. One
  * hgm_batch_run is one graph launch for the whole batch. States must be
- * distinct and match their plan's qubit count and dtype; the plans' compiled
- * modules (at most 16 distinct) are kept loaded while the batch runs, and the
- * graph is captured again when one was unloaded in between. The states and
+ * distinct and match their plan's qubit count and dtype; the plan-compiled
+ * plans' modules (at most 16 distinct) are kept loaded while the batch runs,
+ * and the graph is captured again when one was unloaded in between. The states and
  * plans must outlive the batch. No counterpart in the reference (it runs one
  * state per call: src/core/kernel.cu:192-282). */
 typedef struct hgm_batch hgm_batch;

@@ -686,13 +686,31 @@ int64_t tile_grid(int64_t n_tiles, int block) {
     return g;
 }
 
+// The per-pass kernels' dynamic LDS limit, raised once per kernel (before the
+// first launch, and before any stream capture that launches it).
+template <class T>
+hipError_t pass_lds_attr() {
+    static const hipError_t e =
+        hipFuncSetAttribute((const void *)k_pass_lds<T>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    return e;
+}
+template <class T, int A>
+hipError_t pass_reg_attr() {
+    static const hipError_t e =
+        hipFuncSetAttribute((const void *)k_pass_reg<T, A>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    return e;
+}
+template <class T>
+hipError_t pass_attrs() {
+    for (hipError_t e : {pass_lds_attr<T>(), pass_reg_attr<T, 2>(), pass_reg_attr<T, 4>(), pass_reg_attr<T, 8>(),
+                         pass_reg_attr<T, 16>(), pass_reg_attr<T, 32>()})
+        if (e != hipSuccess) return e;
+    return hipSuccess;
+}
+
 template <class T>
 hipError_t pass_lds(void *state, const DevOp *ops, int n_ops, const PassArgs &a, hipStream_t s) {
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipFuncSetAttribute((const void *)k_pass_lds<T>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_set = true;
-    }
+    pass_lds_attr<T>();
     const size_t lds = 3 * 64 * sizeof(uint64_t) + (size_t(1) << a.r) * sizeof(typename Vec<T>::type);
     const int64_t grid = tile_grid(a.n_tiles, a.B);
     hipLaunchKernelGGL(k_pass_lds<T>, dim3((unsigned)grid), dim3(a.B), lds, s,
@@ -703,11 +721,7 @@ hipError_t pass_lds(void *state, const DevOp *ops, int n_ops, const PassArgs &a,
 template <class T, int A>
 hipError_t pass_reg(void *state, const RegOp *ops, int n_ops, const PhaseMap *maps, const PassMem *pm,
                     uint64_t nonlocal, int64_t n_tiles, int r, int B, size_t lds, hipStream_t s) {
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipFuncSetAttribute((const void *)k_pass_reg<T, A>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_set = true;
-    }
+    pass_reg_attr<T, A>();
     const int64_t grid = tile_grid(n_tiles, B);
     hipLaunchKernelGGL((k_pass_reg<T, A>), dim3((unsigned)grid), dim3(B), lds, s,
                        reinterpret_cast<typename Vec<T>::type *>(state), ops, n_ops, maps, pm, nonlocal, n_tiles, r);
@@ -765,6 +779,8 @@ hipError_t launch_pass_reg(int dtype, int A, void *state, const RegOp *ops, int 
 hipError_t launch_pass_lds(int dtype, void *state, const DevOp *ops, int n_ops, const PassArgs &a, hipStream_t s) {
     return dtype == HGM_C64 ? pass_lds<float>(state, ops, n_ops, a, s) : pass_lds<double>(state, ops, n_ops, a, s);
 }
+
+hipError_t prepare_pass_kernels(int dtype) { return dtype == HGM_C64 ? pass_attrs<float>() : pass_attrs<double>(); }
 
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
     // one 16-byte element per thread over the whole buffer: a flat grid keeps

@@ -20,6 +20,11 @@ struct PassArgs {
 // the pass's gates are applied there, and the tile is written back.
 hipError_t launch_pass_lds(int dtype, void *state, const DevOp *ops, int n_ops, const PassArgs &a, hipStream_t s);
 
+// Raises the dynamic LDS limit of the LDS and register pass kernels of one
+// dtype (otherwise done on their first launch): call before capturing their
+// launches into a graph.
+hipError_t prepare_pass_kernels(int dtype);
+
 // One full-state pass of the register kernel (A amplitudes per thread, B
 // threads per tile of 2^r); lds_bytes is 0 for a pass without remaps.
 hipError_t launch_pass_reg(int dtype, int A, void *state, const RegOp *ops, int n_ops, const PhaseMap *maps,

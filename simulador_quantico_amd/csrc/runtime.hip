@@ -54,7 +54,7 @@ struct hgm_plan {
     }
 };
 
-// A batch of states advanced by plan-compiled plans as one captured graph
+// A batch of states advanced by their plans as one captured graph
 // (hgm_batch_create): branch[i] is the fork branch of state i. 64 states of
 // 12-19 qubits (Grover, 1016 passes) replay in 3.78 ms on 4 branches, 3.14 ms
 // on 16 and 2.91 ms on 64, against 7.1 ms plan by plan.
@@ -67,6 +67,8 @@ struct hgm_batch {
     hipStream_t streams[kBatchBranches] = {};
     hipEvent_t fork = nullptr, join[kBatchBranches] = {};
     std::vector<uint64_t> gens;   // load generation of each plan's module when the graph was captured
+    void *tables = nullptr;       // device copies of the register / LDS plans' tables
+    std::vector<int64_t> table_off;   // per state: byte offset of its plan's tables (-1: plan-compiled)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void drop_graph() {
@@ -346,6 +348,45 @@ size_t plan_table_bytes(const Plan &p) {
     return align256(p.ops.size() * sizeof(DevOp));
 }
 
+// Device tables of a register or LDS plan: where its kernels read their
+// programs (unused by plan-compiled plans).
+struct PlanTables {
+    const void *rops, *maps, *mems;
+    const DevOp *dops;
+};
+
+// Launches every non-empty pass of the plan on `stream`, with no events or
+// waits (jm: the plan's loaded module, for plan-compiled plans). Assumes the
+// lock.
+int launch_passes_locked(hgm_state *s, const Plan &p, JitModule *jm, const PlanTables &t, hipStream_t stream) {
+    const bool reg = p.kernel == HGM_KERNEL_REG;
+    const bool jit = p.kernel == HGM_KERNEL_JIT;
+    PassArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.r = p.shape.r; a.c = p.shape.c; a.B = p.shape.B; a.R = p.shape.R;
+    a.n_tiles = int64_t(1) << (p.qubits - p.shape.r);
+    const uint64_t all = (uint64_t(1) << p.qubits) - 1;
+    const size_t tile_bytes = (size_t(1) << p.shape.r) * amp_bytes(p.dtype);
+    for (size_t pi = 0; pi < p.passes.size(); ++pi) {
+        const Pass &ps = p.passes[pi];
+        if (ps.op_count == 0) continue;
+        a.nonlocal_mask = all & ~ps.local_mask;
+        if (jit) {
+            if (int rc = launch_jit_pass_locked(s, p, jm, pi, stream)) return rc;
+        } else if (reg) {
+            HGM_HIP(launch_pass_reg(p.dtype, 2 * p.shape.R, s->d, static_cast<const RegOp *>(t.rops) + ps.rop_begin,
+                                    (int)ps.rop_count, static_cast<const PhaseMap *>(t.maps),
+                                    static_cast<const PassMem *>(t.mems) + ps.mem, a.nonlocal_mask, a.n_tiles,
+                                    p.shape.r, p.shape.B, ps.phases > 1 ? tile_bytes : 0, stream),
+                    "pass launch");
+        } else {
+            std::memcpy(a.local_pos, ps.local_pos, sizeof(a.local_pos));
+            HGM_HIP(launch_pass_lds(p.dtype, s->d, t.dops + ps.op_begin, (int)ps.op_count, a, stream), "pass launch");
+        }
+    }
+    return HGM_OK;
+}
+
 // Launches every pass of the plan on the context stream. Assumes the lock.
 // With `workspace` (at least plan_table_bytes(p) bytes of device memory) the
 // register and LDS kernels read their tables from there instead of from the
@@ -399,32 +440,8 @@ int run_plan_locked(hgm_state *s, const Plan &p, hgm_stats *st, double plan_ms, 
         HGM_HIP(hipMemcpyAsync(dops, p.ops.data(), p.ops.size() * sizeof(DevOp), hipMemcpyHostToDevice, c.stream),
                 "hipMemcpyAsync(op tables)");
     }
-    PassArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.r = p.shape.r; a.c = p.shape.c; a.B = p.shape.B; a.R = p.shape.R;
-    a.n_tiles = int64_t(1) << (p.qubits - p.shape.r);
-    const uint64_t all = (uint64_t(1) << p.qubits) - 1;
-    const size_t tile_bytes = (size_t(1) << p.shape.r) * amp_bytes(p.dtype);
-    // grid-stride over tiles; a dispatch holds at most 2^32 - 1 work-items
-    const unsigned grid = (unsigned)std::min<int64_t>({a.n_tiles, int64_t(1) << 22, (int64_t)(0xffffffffu / (unsigned)a.B)});
     HGM_HIP(hipEventRecord(c.e0, c.stream), "hipEventRecord");
-    for (size_t pi = 0; pi < p.passes.size(); ++pi) {
-        const Pass &ps = p.passes[pi];
-        if (ps.op_count == 0) continue;
-        a.nonlocal_mask = all & ~ps.local_mask;
-        if (jit) {
-            if ((rc = launch_jit_pass_locked(s, p, jm, pi))) return rc;
-        } else if (reg) {
-            HGM_HIP(launch_pass_reg(p.dtype, 2 * p.shape.R, s->d, static_cast<const RegOp *>(rops) + ps.rop_begin,
-                                    (int)ps.rop_count, static_cast<const PhaseMap *>(maps),
-                                    static_cast<const PassMem *>(mems) + ps.mem, a.nonlocal_mask, a.n_tiles,
-                                    p.shape.r, p.shape.B, ps.phases > 1 ? tile_bytes : 0, c.stream),
-                    "pass launch");
-        } else {
-            std::memcpy(a.local_pos, ps.local_pos, sizeof(a.local_pos));
-            HGM_HIP(launch_pass_lds(p.dtype, s->d, dops + ps.op_begin, (int)ps.op_count, a, c.stream), "pass launch");
-        }
-    }
+    if ((rc = launch_passes_locked(s, p, jm, {rops, maps, mems, dops}, c.stream))) return rc;
     HGM_HIP(hipEventRecord(c.e1, c.stream), "hipEventRecord");
     HGM_HIP(hipEventSynchronize(c.e1), "pass execution");
     add_stats(st, p, elapsed(c.e0, c.e1), plan_ms);
@@ -459,10 +476,28 @@ int prob_halves_locked(const hgm_state *s, int bit, double *zero, double *one) {
 int batch_modules_locked(hgm_batch *b, std::vector<JitModule *> *jms, bool *stale) {
     jms->assign(b->plans.size(), nullptr);
     for (size_t i = 0; i < b->plans.size(); ++i)
-        if (int rc = jit_load_locked(*b->plans[i]->plan.jit, &(*jms)[i])) return rc;
+        if (b->plans[i]->plan.kernel == HGM_KERNEL_JIT)
+            if (int rc = jit_load_locked(*b->plans[i]->plan.jit, &(*jms)[i])) return rc;
     *stale = !b->exec || b->gens.size() != b->plans.size();
-    for (size_t i = 0; i < b->plans.size() && !*stale; ++i) *stale = b->gens[i] != (*jms)[i]->gen;
+    for (size_t i = 0; i < b->plans.size() && !*stale; ++i) *stale = (*jms)[i] && b->gens[i] != (*jms)[i]->gen;
     return HGM_OK;
+}
+
+// Where the kernels of state i's register or LDS plan read their tables: the
+// batch's device copy, laid out as plan_table_bytes counts it.
+PlanTables batch_tables(const hgm_batch *b, size_t i) {
+    PlanTables t{nullptr, nullptr, nullptr, nullptr};
+    if (b->table_off[i] < 0) return t;
+    const Plan &p = b->plans[i]->plan;
+    char *base = static_cast<char *>(b->tables) + b->table_off[i];
+    if (p.kernel == HGM_KERNEL_REG) {
+        t.rops = base;
+        t.maps = base + align256(p.rops.size() * sizeof(RegOp));
+        t.mems = static_cast<const char *>(t.maps) + align256(p.maps.size() * sizeof(PhaseMap));
+    } else {
+        t.dops = reinterpret_cast<const DevOp *>(base);
+    }
+    return t;
 }
 
 // Captures the batch: a fork on the context stream, each state's passes in
@@ -474,12 +509,8 @@ int capture_batch_locked(hgm_batch *b, const std::vector<JitModule *> &jms) {
     int rc = HGM_OK;
     hipError_t e = hipEventRecord(b->fork, c.stream);
     for (int k = 0; k < b->n_branches && e == hipSuccess; ++k) e = hipStreamWaitEvent(b->streams[k], b->fork, 0);
-    for (size_t i = 0; i < b->plans.size() && e == hipSuccess && !rc; ++i) {
-        const Plan &p = b->plans[i]->plan;
-        for (size_t pi = 0; pi < p.passes.size() && !rc; ++pi)
-            if (p.passes[pi].op_count)
-                rc = launch_jit_pass_locked(b->states[i], p, jms[i], pi, b->streams[b->branch[i]]);
-    }
+    for (size_t i = 0; i < b->plans.size() && e == hipSuccess && !rc; ++i)
+        rc = launch_passes_locked(b->states[i], b->plans[i]->plan, jms[i], batch_tables(b, i), b->streams[b->branch[i]]);
     for (int k = 0; k < b->n_branches && e == hipSuccess && !rc; ++k) {
         e = hipEventRecord(b->join[k], b->streams[k]);
         if (e == hipSuccess) e = hipStreamWaitEvent(c.stream, b->join[k], 0);
@@ -499,7 +530,7 @@ int capture_batch_locked(hgm_batch *b, const std::vector<JitModule *> &jms) {
     b->graph = g;
     b->exec = x;
     b->gens.resize(b->plans.size());
-    for (size_t i = 0; i < b->plans.size(); ++i) b->gens[i] = jms[i]->gen;
+    for (size_t i = 0; i < b->plans.size(); ++i) b->gens[i] = jms[i] ? jms[i]->gen : 0;
     return HGM_OK;
 }
 
@@ -1220,10 +1251,7 @@ int hgm_batch_create(hgm_state *const *states, const hgm_plan *const *plans, int
     for (int i = 0; i < n; ++i) {
         if (!states[i] || !plans[i]) { t_err = "null state or plan"; return HGM_E_ARG; }
         const Plan &p = plans[i]->plan;
-        if (p.kernel != HGM_KERNEL_JIT || !p.jit) {
-            t_err = "a batch graph runs plan-compiled plans (HGM_KERNEL_JIT)";
-            return HGM_E_UNSUPPORTED;
-        }
+        if (p.kernel == HGM_KERNEL_JIT && !p.jit) { t_err = "plan-compiled plan without code"; return HGM_E_ARG; }
         if (states[i]->qubits != p.qubits || states[i]->dtype != p.dtype) {
             t_err = "plan was built for another qubit count or dtype";
             return HGM_E_STATE;
@@ -1233,7 +1261,8 @@ int hgm_batch_create(hgm_state *const *states, const hgm_plan *const *plans, int
             return HGM_E_ARG;
         }
         seen.push_back(states[i]->d);
-        if (std::find(mods.begin(), mods.end(), p.jit->hash) == mods.end()) mods.push_back(p.jit->hash);
+        if (p.kernel == HGM_KERNEL_JIT && std::find(mods.begin(), mods.end(), p.jit->hash) == mods.end())
+            mods.push_back(p.jit->hash);
     }
     if (mods.size() > kMaxJitModules) {
         t_err = "a batch graph holds at most " + std::to_string(kMaxJitModules) + " distinct compiled modules";
@@ -1265,6 +1294,7 @@ int hgm_batch_create(hgm_state *const *states, const hgm_plan *const *plans, int
     if (rc) return rc;
     auto release = [&] {
         b->drop_graph();
+        if (b->tables) { hipFree(b->tables); b->tables = nullptr; }
         for (hipStream_t &st : b->streams)
             if (st) { hipStreamDestroy(st); st = nullptr; }
         for (hipEvent_t &ev : b->join)
@@ -1276,8 +1306,41 @@ int hgm_batch_create(hgm_state *const *states, const hgm_plan *const *plans, int
         e = hipStreamCreateWithFlags(&b->streams[k], hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming);
     }
+    // one device copy of the tables of each distinct register or LDS plan
+    b->table_off.assign(n, -1);
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const Plan &p = plans[i]->plan;
+        if (p.kernel == HGM_KERNEL_JIT) continue;
+        int j = 0;
+        while (j < i && plans[j] != plans[i]) ++j;
+        if (j < i) { b->table_off[i] = b->table_off[j]; continue; }
+        b->table_off[i] = (int64_t)total;
+        total += align256(plan_table_bytes(p));
+        if (e == hipSuccess) e = prepare_pass_kernels(p.dtype);
+    }
+    if (e == hipSuccess && total) e = hipMalloc(&b->tables, total);
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        const Plan &p = plans[i]->plan;
+        if (b->table_off[i] < 0 || (i > 0 && std::find(plans, plans + i, plans[i]) != plans + i)) continue;
+        const PlanTables t = batch_tables(b.get(), (size_t)i);
+        if (p.kernel == HGM_KERNEL_REG) {
+            if (!p.rops.empty())
+                e = hipMemcpy(const_cast<void *>(t.rops), p.rops.data(), p.rops.size() * sizeof(RegOp),
+                              hipMemcpyHostToDevice);
+            if (e == hipSuccess && !p.maps.empty())
+                e = hipMemcpy(const_cast<void *>(t.maps), p.maps.data(), p.maps.size() * sizeof(PhaseMap),
+                              hipMemcpyHostToDevice);
+            if (e == hipSuccess && !p.mems.empty())
+                e = hipMemcpy(const_cast<void *>(t.mems), p.mems.data(), p.mems.size() * sizeof(PassMem),
+                              hipMemcpyHostToDevice);
+        } else if (!p.ops.empty()) {
+            e = hipMemcpy(const_cast<DevOp *>(t.dops), p.ops.data(), p.ops.size() * sizeof(DevOp),
+                          hipMemcpyHostToDevice);
+        }
+    }
     if (e != hipSuccess) {
-        rc = hip_fail(e, "batch streams");
+        rc = hip_fail(e, "batch streams and tables");
         release();
         return rc;
     }
@@ -1319,6 +1382,7 @@ void hgm_batch_destroy(hgm_batch *b) {
     std::lock_guard<std::mutex> g(ctx().mu);
     if (ctx().stream) hipStreamSynchronize(ctx().stream);
     b->drop_graph();
+    if (b->tables) hipFree(b->tables);
     for (hipStream_t st : b->streams)
         if (st) hipStreamDestroy(st);
     for (hipEvent_t ev : b->join)

@@ -408,9 +408,10 @@ class Plan:
 
 
 class Batch:
-    """States of any qubit counts, each advanced by its plan-compiled plan,
-    replayed as one captured hipGraph (hgm_batch_*): run() is one graph launch
-    for the whole batch. Keeps references to the states and plans."""
+    """States of any qubit counts, each advanced by its plan (any kernel
+    family), replayed as one captured hipGraph (hgm_batch_*): run() is one
+    graph launch for the whole batch. Keeps references to the states and
+    plans."""
 
     def __init__(self, states, plans):
         if len(states) != len(plans):

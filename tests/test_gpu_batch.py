@@ -1,8 +1,9 @@
 """The batch graph (hgm_batch_*): resident states of uneven qubit counts, each
-advanced by its plan-compiled plan, captured as one hipGraph with parallel
-branches. Every replay gives each state exactly what running its plan alone
-gives (same kernels, same launch shapes, so bit-identical), with eager work on
-the states between replays."""
+advanced by its plan (plan-compiled, interpreted register kernel or LDS
+kernel), captured as one hipGraph with parallel branches. Every replay gives
+each state exactly what running its plan alone gives (same kernels, same
+launch shapes, so bit-identical), with eager work on the states between
+replays."""
 import sys
 
 import numpy as np
@@ -24,10 +25,14 @@ def _ops(a):
     return (N.Op * len(a)).from_buffer_copy(a.tobytes())
 
 
-def _jit_plan(n, ops, dtype):
+def _jit_plan(n, ops, dtype, kernel=N.HGM_KERNEL_JIT):
     c = 4 if dtype == np.complex64 else 3
-    t = N.default_tuning(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=c,
-                         kernel=N.HGM_KERNEL_JIT, planner=N.HGM_PLAN_GREEDY | 32, ops_block=1 << 30)
+    if kernel == N.HGM_KERNEL_LDS:
+        t = N.default_tuning(block_size=256, repeat_count=4, gpu_region_bits=11, gpu_coalesced_bits=c,
+                             kernel=kernel, planner=N.HGM_PLAN_GREEDY)
+    else:
+        t = N.default_tuning(block_size=256, repeat_count=8, gpu_region_bits=12, gpu_coalesced_bits=c,
+                             kernel=kernel, planner=N.HGM_PLAN_GREEDY | 32, ops_block=1 << 30)
     return hq.Plan(n, _ops(ops), t, 1, dtype)
 
 
@@ -42,7 +47,11 @@ def test_batch_graph_matches_each_plan_alone(dtype):
             S.build_job(5, 0.0, "grover", 18, rng, grover_max_iter=8),
             S.build_job(6, 0.0, "random", 19, rng, random_depth=4),
             S.build_job(7, 0.0, "grover", 20, rng, grover_max_iter=8)]
-    plans = [_jit_plan(j.qubits, j.ops, dtype) for j in jobs]
+    # plan-compiled plans, with interpreted register-kernel plans (the random layers) and an LDS plan (the adder)
+    kernels = {2: N.HGM_KERNEL_REG, 4: N.HGM_KERNEL_LDS, 6: N.HGM_KERNEL_REG}
+    plans = [_jit_plan(j.qubits, j.ops, dtype, kernels.get(j.index, N.HGM_KERNEL_JIT)) for j in jobs]
+    assert [p.kernel for p in plans][2:7] == [N.HGM_KERNEL_REG, N.HGM_KERNEL_JIT, N.HGM_KERNEL_LDS,
+                                              N.HGM_KERNEL_JIT, N.HGM_KERNEL_REG]
     plans.append(plans[5])                     # one plan shared by two states
     jobs.append(jobs[5])
     say("batch graph:", [(j.kind, j.qubits) for j in jobs], np.dtype(dtype).name)
@@ -83,12 +92,9 @@ def test_batch_graph_rejects_what_it_cannot_capture():
         hq.Batch([s1, hq.State(15, np.complex64)], [p, p])
     with pytest.raises(N.HGMError, match="another qubit count or dtype"):
         hq.Batch([hq.State(14, np.complex128)], [p])
-    t = N.default_tuning(kernel=N.HGM_KERNEL_REG)
-    reg = hq.Plan(14, _ops(ops), t, 1, np.complex64)
-    with pytest.raises(N.HGMError, match="plan-compiled"):
-        hq.Batch([s1, s2], [p, reg])
-    b = hq.Batch([s1, s2], [p, p])             # a valid batch still builds and runs after the failures
+    reg = _jit_plan(14, ops, np.complex64, N.HGM_KERNEL_REG)
+    b = hq.Batch([s1, s2], [p, reg])           # a valid batch still builds and runs after the failures
     s1.set_basis(0)
     s2.set_basis(0)
     b.run()
-    assert np.array_equal(s1.download(), s2.download())
+    assert np.abs(s1.download() - s2.download()).max() < 1e-6   # same circuit, two kernel families
