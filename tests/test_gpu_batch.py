@@ -98,3 +98,32 @@ def test_batch_graph_rejects_what_it_cannot_capture():
     s2.set_basis(0)
     b.run()
     assert np.abs(s1.download() - s2.download()).max() < 1e-6   # same circuit, two kernel families
+
+
+def test_batch_graph_beyond_64_states_and_of_one():
+    """70 states share the 64 branches (some branches run two states in
+    order); a batch of one state is one branch. Results equal the plans run
+    alone."""
+    rng = np.random.default_rng(62)
+    job = S.build_job(0, 0.0, "grover", 12, rng, grover_max_iter=4)
+    jit = _jit_plan(12, job.ops, np.complex64)
+    reg = _jit_plan(12, job.ops, np.complex64, N.HGM_KERNEL_REG)
+    ref = {}
+    for name, p in (("jit", jit), ("reg", reg)):
+        s = hq.State(12, np.complex64)
+        s.set_basis(0)
+        p.run(s)
+        ref[name] = s.download()
+    plans = [jit if k % 3 else reg for k in range(70)]
+    states = [hq.State(12, np.complex64) for _ in plans]
+    batch = hq.Batch(states, plans)
+    for replay in range(2):
+        for s in states:
+            s.set_basis(0)
+        batch.run()
+        for k, s in enumerate(states):
+            assert np.array_equal(s.download(), ref["jit" if k % 3 else "reg"]), (replay, k)
+    one = hq.Batch([states[0]], [reg])
+    states[0].set_basis(0)
+    one.run()
+    assert np.array_equal(states[0].download(), ref["reg"])
